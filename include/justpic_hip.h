@@ -1,0 +1,172 @@
+/*
+ * justpic_hip.h -- C-ABI of the MI355X (gfx950) particle hot-path library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (JP_OK == 0, negative on error; jp_last_error() gives the text) and
+ * throws nothing across the ABI.  Array arguments are DEVICE pointers unless the
+ * comment says otherwise; arrays of pointers (`double* const* coords`) are HOST
+ * arrays whose elements are device pointers.
+ *
+ * Storage layout (identical to JustPIC's GPU CellArrays, block length B = 0):
+ *   a per-particle quantity with K = max_xcell slots per cell over C = prod(ni)
+ *   cells is one contiguous array of C*K elements, element (cell c, slot k) at
+ *   data[c + C*k]; c = i + ni[0]*(j + ni[1]*l) is the column-major (Julia) cell
+ *   index, all indices 0-based.  The active-slot mask ("index") is one byte per
+ *   slot (Julia Bool).  Nodal fields are column-major with dims ni[d]+1.
+ *
+ * Reference: JuliaGeodynamics/JustPIC.jl v0.6.7.  Each function names the
+ * reference interface it replaces.
+ */
+#ifndef JUSTPIC_HIP_H
+#define JUSTPIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JP_OK 0
+#define JP_ERR_ARG (-1)      /* invalid argument (sizes, dims, method, alpha) */
+#define JP_ERR_HIP (-2)      /* HIP runtime failure */
+#define JP_ERR_STATE (-3)    /* context misuse */
+
+#define JP_MAX_ARGS 16       /* particle fields moved together with the coordinates */
+
+/* integrators (src/Advection/types.jl:113-145) */
+#define JP_EULER 0
+#define JP_RK2 1
+#define JP_RK4 2
+
+/* One grid axis: materialised coordinates plus spacing and inverse spacing.
+ * For a range grid the reference stores a scalar spacing x[2]-x[1]
+ * (src/Particles/particles_utils.jl:137-146); the caller replicates it into
+ * dx[0..n-2].  For a vector grid the reference stores diff(x)
+ * (src/Particles/particles_utils.jl:76-85).  inv_dx[k] must equal 1/dx[k]. */
+typedef struct jp_axis {
+    const double* x;      /* n points, strictly increasing */
+    const double* dx;     /* n-1 spacings  (particles.di.*)  */
+    const double* inv_dx; /* n-1 inverses  (particles._di.*) */
+    int64_t n;
+} jp_axis;
+
+/* Cell grid of a Particles container: particles.xvi + di.vertex + _di.vertex,
+ * cells ni = size(particles.index), K = particles.max_xcell. */
+typedef struct jp_cellgrid {
+    int32_t ndim;        /* 2 or 3 */
+    int32_t max_xcell;   /* K, 1..64 */
+    int64_t ni[3];       /* cells per dimension (ni[2] ignored in 2D) */
+    jp_axis xv[3];       /* vertex axes, xv[d].n == ni[d] + 1 */
+} jp_cellgrid;
+
+/* Staggered velocity grids: particles.xi_vel[c][d] with di.velocity[c][d]. */
+typedef struct jp_velgrid {
+    jp_axis ax[3][3];    /* ax[c][d]: axis d of velocity component c */
+} jp_velgrid;
+
+/* Data-dependent outcomes the reference handles silently (move_safe.jl:95-117,
+ * injection.jl:119-123).  Counts accumulate over the calls they are passed to. */
+typedef struct jp_stats {
+    int64_t moved;     /* particles re-inserted into a free slot              */
+    int64_t deleted;   /* particles removed for leaving the domain            */
+    int64_t dropped;   /* movers lost because the destination cell was full   */
+    int64_t far;       /* movers whose destination is not a neighbouring cell */
+    int64_t injected;  /* particles created by injection                      */
+    int64_t orphan;    /* injected particles with no neighbour to copy from   */
+} jp_stats;
+
+typedef struct jp_ctx jp_ctx;
+
+/* ---- context, memory, streams (no reference counterpart: they replace the
+ *      AMDGPU.jl allocation the reference ext/JustPICAMDGPUExt.jl:26-47 uses) */
+int jp_ctx_create(jp_ctx** out, int device);
+int jp_ctx_destroy(jp_ctx* ctx);
+const char* jp_last_error(const jp_ctx* ctx);
+/* 1 (default): every entry point synchronises its stream before returning, as
+ * the reference launch! does (src/launch.jl:65-69).  0: asynchronous. */
+int jp_ctx_set_sync(jp_ctx* ctx, int synchronous);
+/* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
+int jp_ctx_set_stream(jp_ctx* ctx, void* hip_stream);
+void* jp_ctx_stream(jp_ctx* ctx);
+int jp_sync(jp_ctx* ctx);
+int jp_malloc(jp_ctx* ctx, void** dptr, size_t bytes);
+int jp_free(jp_ctx* ctx, void* dptr);
+int jp_memcpy_h2d(jp_ctx* ctx, void* dst, const void* host_src, size_t bytes);
+int jp_memcpy_d2h(jp_ctx* ctx, void* host_dst, const void* src, size_t bytes);
+int jp_memcpy_d2d(jp_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* fill with a byte value, or with a double value (NaN planes, zero fields) */
+int jp_fill_u8(jp_ctx* ctx, uint8_t* dptr, uint8_t value, int64_t count);
+int jp_fill_f64(jp_ctx* ctx, double* dptr, double value, int64_t count);
+int jp_fill_f32(jp_ctx* ctx, float* dptr, float value, int64_t count);
+
+/* timing on the context stream (hipEvents) */
+int jp_event_create(jp_ctx* ctx, void** ev);
+int jp_event_destroy(jp_ctx* ctx, void* ev);
+int jp_event_record(jp_ctx* ctx, void* ev);
+int jp_event_elapsed_ms(jp_ctx* ctx, void* ev_start, void* ev_stop, float* ms);
+
+/* hipGraph capture of whatever entry points are called between begin and end
+ * on the context stream (asynchronous mode is forced while capturing). */
+int jp_graph_begin(jp_ctx* ctx);
+int jp_graph_end(jp_ctx* ctx, void** graph_exec);
+int jp_graph_launch(jp_ctx* ctx, void* graph_exec);
+int jp_graph_destroy(jp_ctx* ctx, void* graph_exec);
+
+/* ---- hot path, FP64 drop-in ------------------------------------------- */
+
+/* init_particles, random quadrant-stratified fill
+ * (src/Particles/particles_utils.jl:108-194, fill_coords_index! :168-194).
+ * rand() is replaced by the Philox4x32-10 stream (seed, cell, slot, dim).
+ * coords[d] and index must be pre-sized C*K; every slot is written. */
+int jp_init_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                      uint8_t* index, int np_quadrant, uint64_t seed);
+
+/* advection!(particles, method, V, grid_vi, dt, dxi)
+ * (src/Particles/Advection/advection.jl:35-91, RK2.jl:1-26, RK4.jl:1-19,
+ * Euler.jl:1-20).  V[c] is column-major with dims vg->ax[c][d].n. */
+int jp_advection(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                 const uint8_t* index, const double* const* V,
+                 const jp_velgrid* vg, int method, double alpha, double dt);
+
+/* move_particles!(particles, grid, args, dxi)
+ * (src/Particles/move_safe.jl:23-125).  stats may be NULL (no host sync). */
+int jp_move_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                      uint8_t* index, int nargs, double* const* args,
+                      jp_stats* stats);
+
+/* particle2grid!(F, Fp, particles) for one scalar field
+ * (src/Interpolations/particle_to_grid.jl:23-68 2D, :113-151 3D). */
+int jp_particle2grid(jp_ctx* ctx, const jp_cellgrid* g, double* F,
+                     const double* Fp, const double* const* coords,
+                     const uint8_t* index);
+
+/* particle2grid!(F::NTuple, Fp::NTuple, particles)
+ * (src/Interpolations/particle_to_grid.jl:70-109 2D, :153-199 3D). */
+int jp_particle2grid_multi(jp_ctx* ctx, const jp_cellgrid* g, int nfields,
+                           double* const* F, const double* const* Fp,
+                           const double* const* coords, const uint8_t* index);
+
+/* grid2particle!(Fp, xvi, F, particles, di)
+ * (src/Interpolations/grid_to_particle.jl:26-82, _grid2particle :265-273). */
+int jp_grid2particle(jp_ctx* ctx, const jp_cellgrid* g, double* Fp,
+                     const double* F, const double* const* coords,
+                     const uint8_t* index);
+
+/* inject_particles!(particles, args, grid, di)
+ * (src/Particles/injection.jl:21-131, index_min_distance :330-393,
+ * new_particle :411-417).  rand() is replaced by Philox4x32-10 keyed by
+ * (seed) with counter (cell, slot, dim-block, call_counter). */
+int jp_inject_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                        uint8_t* index, int nargs, double* const* args,
+                        int min_xcell, uint64_t seed, uint64_t call_counter,
+                        jp_stats* stats);
+
+/* Philox4x32-10 block (host helper, for tests): out[4] <- philox(ctr[4], key[2]) */
+void jp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JUSTPIC_HIP_H */

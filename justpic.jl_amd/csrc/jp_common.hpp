@@ -1,0 +1,170 @@
+// jp_common.hpp -- device-side building blocks shared by the gfx950 kernels.
+//
+// All floating-point code here is compiled with -ffp-contract=off: every fused
+// multiply-add is an explicit fma() placed where JustPIC.jl fuses (Base.fma,
+// Base.muladd, MuladdMacro.@muladd); every other operation is one correctly
+// rounded binary64 (or binary32 in the split kernels) operation in the
+// reference's evaluation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "justpic_hip.h"
+
+namespace jp {
+
+constexpr int kBlock = 256;
+
+struct Axis {
+    const double* x;
+    const double* dx;
+    const double* idx;  // inverse spacing
+    int64_t n;
+};
+
+template <int D>
+struct Grid {
+    int64_t ni[D];
+    int64_t C;  // cells
+    int K;      // slots per cell
+    Axis xv[D];
+};
+
+template <int D>
+struct Vel {
+    Axis ax[D][D];
+};
+
+struct Ptr3 {
+    double* p[3];
+};
+struct CPtr3 {
+    const double* p[3];
+};
+struct ArgPtrs {
+    double* p[JP_MAX_ARGS];
+};
+
+// 0-based cell tuple <-> column-major linear index
+template <int D>
+__host__ __device__ inline int64_t lin(const int64_t (&ni)[D], const int64_t* I) {
+    int64_t c = I[D - 1];
+#pragma unroll
+    for (int d = D - 2; d >= 0; --d) c = c * ni[d] + I[d];
+    return c;
+}
+
+template <int D>
+__host__ __device__ inline void unlin(const int64_t (&ni)[D], int64_t c, int64_t* I) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I[d] = c % ni[d];
+        c /= ni[d];
+    }
+}
+
+// lerp(t, v0, v1) = muladd(t, v1, muladd(-t, v0, v0)) -- src/Interpolations/ndlerp.jl:15
+__host__ __device__ inline double lerp1(double t, double v0, double v1) {
+    return fma(t, v1, fma(-t, v0, v0));
+}
+__host__ __device__ inline float lerp1f(float t, float v0, float v1) {
+    return fmaf(t, v1, fmaf(-t, v0, v0));
+}
+
+// find_parent_cell_bisection (src/Utils.jl:108-130), 0-based: lo=0, hi=n-1.
+// The iteration cap only guards memory safety; for p inside [x[0], x[n-1]]
+// the loop terminates in O(log n) steps.  Returns -1 if it did not.
+__device__ inline int64_t bisect_exact(double p, const double* __restrict__ x, int64_t n,
+                                       int64_t s) {
+    int64_t lo = 0, hi = n - 1;
+    for (int it = 0; it < 160; ++it) {
+        double a = x[s], b = x[s + 1];
+        if (a <= p && p <= b) return s;
+        if (a < p) {
+            lo = s;
+            s = (hi + s) >> 1;
+        } else {
+            hi = s;
+            s = (lo + s) >> 1;
+        }
+    }
+    return -1;
+}
+
+// Same result as bisect_exact for any p in [x[0], x[n-1]]: if p lies strictly
+// inside one interval, that interval is the only one whose closed bracket
+// contains p, so the reference loop must stop there whatever its seed; only
+// ties (p equal to a grid value) depend on the search path, and those run the
+// reference loop itself.  Candidates: the seed, its neighbours, and the index a
+// uniform spacing would give.
+__device__ inline int64_t find_cell(double p, const Axis& a, int64_t seed) {
+    const double* __restrict__ x = a.x;
+    const int64_t last = a.n - 2;
+    int64_t s = seed < last ? seed : last;
+    double xs = x[s], xs1 = x[s + 1];
+    if (xs < p && p < xs1) return s;
+    if (p < xs) {
+        if (s > 0) {
+            double xm = x[s - 1];
+            if (xm < p && p < xs) return s - 1;
+        }
+    } else if (p > xs1 && s < last) {
+        double xp = x[s + 2];
+        if (xs1 < p && p < xp) return s + 1;
+    }
+    double x0 = x[0], xn = x[a.n - 1];
+    double fg = (p - x0) / (xn - x0) * (double)(a.n - 1);
+    int64_t g = (int64_t)fg;
+    g = g < 0 ? 0 : (g > last ? last : g);
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        int64_t h = g + (o == 0 ? 0 : (o == 1 ? -1 : 1));
+        if (h >= 0 && h <= last && x[h] < p && p < x[h + 1]) return h;
+    }
+    int64_t r = bisect_exact(p, x, a.n, s);
+    return r < 0 ? 0 : r;
+}
+
+// ---- Philox4x32-10 (Salmon et al., SC11) --------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+    }
+}
+
+// uniform double in [0,1), 53 bits: counter (cell, slot|dimblock<<8|purpose<<16,
+// counter lo, counter hi), key = seed.  dims 0/1: block 0 words (0,1)/(2,3);
+// dim 2: block 1 words (0,1).
+__host__ __device__ inline double uniform53(uint64_t seed, uint64_t cell, int slot, int dim,
+                                            int purpose, uint64_t counter) {
+    int block = dim >> 1, half = dim & 1;
+    uint32_t c[4];
+    c[0] = (uint32_t)cell;
+    c[1] = (uint32_t)slot | ((uint32_t)block << 8) | ((uint32_t)purpose << 16);
+    c[2] = (uint32_t)counter;
+    c[3] = (uint32_t)(counter >> 32);
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t w = half ? (((uint64_t)c[2] << 32) | c[3]) : (((uint64_t)c[0] << 32) | c[1]);
+    return (double)(w >> 11) * 0x1.0p-53;
+}
+
+__device__ inline void stat_add(jp_stats* st, int field, int64_t v) {
+    if (st && v) atomicAdd(reinterpret_cast<unsigned long long*>(&st->moved) + field,
+                           (unsigned long long)v);
+}
+enum { ST_MOVED = 0, ST_DELETED, ST_DROPPED, ST_FAR, ST_INJECTED, ST_ORPHAN };
+
+}  // namespace jp

@@ -1,0 +1,298 @@
+// jp_ctx.hip -- context, memory, events and graph capture of the C-ABI.
+#include "jp_ctx.hpp"
+
+#include <cstring>
+
+using namespace jp;
+
+namespace {
+
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_f64(double* p, double v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_f32(float* p, float v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int64_t fill_grid(int64_t n) {
+    int64_t b = blocks_for(n);
+    return b < 1 ? 1 : (b > 65536 ? 65536 : b);
+}
+
+}  // namespace
+
+namespace jp {
+
+int ensure_scratch(jp_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->scratch_bytes) return JP_OK;
+    if (ctx->capturing)
+        return fail(ctx, JP_ERR_STATE,
+                    "scratch of %s%lld bytes needed during graph capture; run the "
+                    "captured sequence once before capturing",
+                    "", (long long)bytes);
+    if (ctx->scratch) JP_HIP(ctx, hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    size_t sz = (bytes + 4095) & ~(size_t)4095;
+    JP_HIP(ctx, hipMalloc(&ctx->scratch, sz));
+    ctx->scratch_bytes = sz;
+    return JP_OK;
+}
+
+int zero_bytes(jp_ctx* ctx, void* p, size_t bytes) {
+    if (!bytes) return JP_OK;
+    int64_t n = (int64_t)bytes;
+    k_fill_u8<<<fill_grid(n), kBlock, 0, ctx->stream>>>((uint8_t*)p, 0, n);
+    JP_LAUNCHED(ctx);
+    return JP_OK;
+}
+
+int stats_begin(jp_ctx* ctx, jp_stats* host) {
+    if (!host) return JP_OK;
+    if (ctx->capturing)
+        return fail(ctx, JP_ERR_STATE, "stats%s cannot be returned during graph capture");
+    return zero_bytes(ctx, ctx->dsmall, sizeof(jp_stats));
+}
+
+int stats_end(jp_ctx* ctx, jp_stats* host) {
+    if (!host) return JP_OK;
+    jp_stats tmp;
+    JP_HIP(ctx, hipMemcpyAsync(&tmp, ctx->dsmall, sizeof tmp, hipMemcpyDeviceToHost, ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    host->moved += tmp.moved;
+    host->deleted += tmp.deleted;
+    host->dropped += tmp.dropped;
+    host->far += tmp.far;
+    host->injected += tmp.injected;
+    host->orphan += tmp.orphan;
+    return JP_OK;
+}
+
+int finish(jp_ctx* ctx) {
+    if (ctx->sync && !ctx->capturing) JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JP_OK;
+}
+
+int check_grid(jp_ctx* ctx, const jp_cellgrid* g) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!g) return fail(ctx, JP_ERR_ARG, "null grid%s");
+    if (g->ndim != 2 && g->ndim != 3)
+        return fail(ctx, JP_ERR_ARG, "The dimension of the problem must be either 2 or 3%s (got %lld)",
+                    "", g->ndim);
+    if (g->max_xcell < 1 || g->max_xcell > 64)
+        return fail(ctx, JP_ERR_ARG, "max_xcell must be in 1..64%s (got %lld)", "", g->max_xcell);
+    for (int d = 0; d < g->ndim; ++d) {
+        if (g->ni[d] < 1) return fail(ctx, JP_ERR_ARG, "ni must be positive%s");
+        if (g->xv[d].n != g->ni[d] + 1)
+            return fail(ctx, JP_ERR_ARG, "vertex axis length must be ni+1%s (dim %lld)", "", d);
+        if (!g->xv[d].x || !g->xv[d].dx || !g->xv[d].inv_dx)
+            return fail(ctx, JP_ERR_ARG, "null vertex axis pointer%s (dim %lld)", "", d);
+    }
+    int64_t S = g->max_xcell;
+    for (int d = 0; d < g->ndim; ++d) S *= g->ni[d];
+    if (S > ((int64_t)1 << 40)) return fail(ctx, JP_ERR_ARG, "too many slots%s");
+    return JP_OK;
+}
+
+}  // namespace jp
+
+extern "C" {
+
+int jp_ctx_create(jp_ctx** out, int device) {
+    if (!out) return JP_ERR_ARG;
+    *out = nullptr;
+    jp_ctx* ctx = new jp_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&ctx->dsmall, 4096);
+    if (e == hipSuccess) e = hipMemset(ctx->dsmall, 0, 4096);
+    if (e == hipSuccess) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+            cu > 0)
+            ctx->num_cu = cu;
+    }
+    if (e != hipSuccess) {
+        delete ctx;
+        return JP_ERR_HIP;
+    }
+    ctx->stream = ctx->own;
+    *out = ctx;
+    return JP_OK;
+}
+
+int jp_ctx_destroy(jp_ctx* ctx) {
+    if (!ctx) return JP_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->own) hipStreamSynchronize(ctx->own);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->dsmall) hipFree(ctx->dsmall);
+    if (ctx->own) hipStreamDestroy(ctx->own);
+    delete ctx;
+    return JP_OK;
+}
+
+const char* jp_last_error(const jp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int jp_ctx_set_sync(jp_ctx* ctx, int synchronous) {
+    if (!ctx) return JP_ERR_STATE;
+    ctx->sync = synchronous ? 1 : 0;
+    return JP_OK;
+}
+
+int jp_ctx_set_stream(jp_ctx* ctx, void* s) {
+    if (!ctx) return JP_ERR_STATE;
+    if (ctx->capturing) return fail(ctx, JP_ERR_STATE, "cannot change stream while capturing%s");
+    ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own;
+    return JP_OK;
+}
+
+void* jp_ctx_stream(jp_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int jp_sync(jp_ctx* ctx) {
+    if (!ctx) return JP_ERR_STATE;
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JP_OK;
+}
+
+int jp_malloc(jp_ctx* ctx, void** p, size_t bytes) {
+    if (!ctx || !p) return JP_ERR_ARG;
+    *p = nullptr;
+    if (!bytes) bytes = 1;
+    JP_HIP(ctx, hipMalloc(p, bytes));
+    return JP_OK;
+}
+
+int jp_free(jp_ctx* ctx, void* p) {
+    if (!ctx) return JP_ERR_STATE;
+    if (p) JP_HIP(ctx, hipFree(p));
+    return JP_OK;
+}
+
+int jp_memcpy_h2d(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!bytes) return JP_OK;
+    JP_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JP_OK;
+}
+
+int jp_memcpy_d2h(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!bytes) return JP_OK;
+    JP_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JP_OK;
+}
+
+int jp_memcpy_d2d(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!bytes) return JP_OK;
+    JP_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return finish(ctx);
+}
+
+int jp_fill_u8(jp_ctx* ctx, uint8_t* p, uint8_t v, int64_t n) {
+    if (!ctx) return JP_ERR_STATE;
+    if (n <= 0) return JP_OK;
+    k_fill_u8<<<fill_grid(n), kBlock, 0, ctx->stream>>>(p, v, n);
+    JP_LAUNCHED(ctx);
+    return finish(ctx);
+}
+
+int jp_fill_f64(jp_ctx* ctx, double* p, double v, int64_t n) {
+    if (!ctx) return JP_ERR_STATE;
+    if (n <= 0) return JP_OK;
+    k_fill_f64<<<fill_grid(n), kBlock, 0, ctx->stream>>>(p, v, n);
+    JP_LAUNCHED(ctx);
+    return finish(ctx);
+}
+
+int jp_fill_f32(jp_ctx* ctx, float* p, float v, int64_t n) {
+    if (!ctx) return JP_ERR_STATE;
+    if (n <= 0) return JP_OK;
+    k_fill_f32<<<fill_grid(n), kBlock, 0, ctx->stream>>>(p, v, n);
+    JP_LAUNCHED(ctx);
+    return finish(ctx);
+}
+
+int jp_event_create(jp_ctx* ctx, void** ev) {
+    if (!ctx || !ev) return JP_ERR_ARG;
+    hipEvent_t e;
+    JP_HIP(ctx, hipEventCreate(&e));
+    *ev = e;
+    return JP_OK;
+}
+
+int jp_event_destroy(jp_ctx* ctx, void* ev) {
+    if (!ctx) return JP_ERR_STATE;
+    if (ev) JP_HIP(ctx, hipEventDestroy(static_cast<hipEvent_t>(ev)));
+    return JP_OK;
+}
+
+int jp_event_record(jp_ctx* ctx, void* ev) {
+    if (!ctx || !ev) return JP_ERR_ARG;
+    JP_HIP(ctx, hipEventRecord(static_cast<hipEvent_t>(ev), ctx->stream));
+    return JP_OK;
+}
+
+int jp_event_elapsed_ms(jp_ctx* ctx, void* a, void* b, float* ms) {
+    if (!ctx || !a || !b || !ms) return JP_ERR_ARG;
+    JP_HIP(ctx, hipEventSynchronize(static_cast<hipEvent_t>(b)));
+    JP_HIP(ctx, hipEventElapsedTime(ms, static_cast<hipEvent_t>(a), static_cast<hipEvent_t>(b)));
+    return JP_OK;
+}
+
+int jp_graph_begin(jp_ctx* ctx) {
+    if (!ctx) return JP_ERR_STATE;
+    if (ctx->capturing) return fail(ctx, JP_ERR_STATE, "already capturing%s");
+    if (ctx->stream == nullptr)
+        return fail(ctx, JP_ERR_STATE, "graph capture needs a non-null stream%s");
+    JP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return JP_OK;
+}
+
+int jp_graph_end(jp_ctx* ctx, void** exec) {
+    if (!ctx || !exec) return JP_ERR_ARG;
+    if (!ctx->capturing) return fail(ctx, JP_ERR_STATE, "not capturing%s");
+    ctx->capturing = false;
+    hipGraph_t graph;
+    JP_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
+    hipGraphExec_t ge;
+    hipError_t e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    JP_HIP(ctx, e);
+    *exec = ge;
+    return JP_OK;
+}
+
+int jp_graph_launch(jp_ctx* ctx, void* exec) {
+    if (!ctx || !exec) return JP_ERR_ARG;
+    JP_HIP(ctx, hipGraphLaunch(static_cast<hipGraphExec_t>(exec), ctx->stream));
+    return finish(ctx);
+}
+
+int jp_graph_destroy(jp_ctx* ctx, void* exec) {
+    if (!ctx) return JP_ERR_STATE;
+    if (exec) JP_HIP(ctx, hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec)));
+    return JP_OK;
+}
+
+void jp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+}  // extern "C"

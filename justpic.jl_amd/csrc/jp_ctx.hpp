@@ -1,0 +1,78 @@
+// jp_ctx.hpp -- host-side context shared by the entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "jp_common.hpp"
+
+struct jp_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    int sync = 1;
+    bool capturing = false;
+    std::string err;
+    // small device area: [0] jp_stats, [64..] int flags
+    void* dsmall = nullptr;
+    // growable scratch (never reallocated during capture)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int num_cu = 256;
+};
+
+namespace jp {
+
+inline int fail(jp_ctx* ctx, int code, const char* fmt, const char* a = "", long long b = 0) {
+    if (ctx) {
+        char buf[512];
+        snprintf(buf, sizeof buf, fmt, a, b);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define JP_HIP(ctx, call)                                                          \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return ::jp::fail((ctx), JP_ERR_HIP, "HIP error %s (line %lld)",       \
+                              hipGetErrorString(e_), (long long)__LINE__);         \
+    } while (0)
+
+#define JP_LAUNCHED(ctx) JP_HIP(ctx, hipGetLastError())
+
+inline jp_stats* dstats(jp_ctx* ctx) { return reinterpret_cast<jp_stats*>(ctx->dsmall); }
+inline int* dflags(jp_ctx* ctx) {
+    return reinterpret_cast<int*>(static_cast<char*>(ctx->dsmall) + 256);
+}
+
+// scratch of at least `bytes`; grows outside capture only
+int ensure_scratch(jp_ctx* ctx, size_t bytes);
+// zero `n` ints/bytes with a kernel (no memset nodes inside graphs)
+int zero_bytes(jp_ctx* ctx, void* p, size_t bytes);
+// stats bracket: zero the device stats before, copy + accumulate after
+int stats_begin(jp_ctx* ctx, jp_stats* host);
+int stats_end(jp_ctx* ctx, jp_stats* host);
+int finish(jp_ctx* ctx);  // synchronise when the context is synchronous
+
+inline int64_t blocks_for(int64_t n, int b = kBlock) { return (n + b - 1) / b; }
+
+int check_grid(jp_ctx* ctx, const jp_cellgrid* g);
+
+template <int D>
+Grid<D> to_grid(const jp_cellgrid* g) {
+    Grid<D> G;
+    G.C = 1;
+    for (int d = 0; d < D; ++d) {
+        G.ni[d] = g->ni[d];
+        G.C *= g->ni[d];
+        G.xv[d] = Axis{g->xv[d].x, g->xv[d].dx, g->xv[d].inv_dx, g->xv[d].n};
+    }
+    G.K = g->max_xcell;
+    return G;
+}
+
+}  // namespace jp

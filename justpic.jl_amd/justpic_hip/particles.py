@@ -1,0 +1,211 @@
+"""Particles container and CellArrays on the HIP device.
+
+Mirrors ``JustPIC.Particles`` (src/particles.jl:17-50) and the GPU
+``CellArray`` layout of the reference's device backends
+(ext/JustPICAMDGPUExt.jl:26-35: block length 0, data shape (C, K, 1)), i.e.
+element (cell c, slot k) at ``data[c + C*k]`` with c the column-major cell
+index.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from .device import Context, DeviceArray
+from .grids import Axis
+
+
+class CellArray:
+    """K slots per cell over ``ni`` cells (B = 0 planes).  ``data`` is the flat
+    device buffer; ``to_host()`` returns a (K, C) numpy array (slot-major)."""
+
+    def __init__(self, ctx: Context, ni, K, dtype=np.float64, fill=None):
+        self.ctx = ctx
+        self.ni = tuple(int(n) for n in ni)
+        self.K = int(K)
+        self.C = int(np.prod(self.ni))
+        self.data = DeviceArray(ctx, (self.C * self.K,), dtype)
+        if fill is not None:
+            self.data.fill(fill)
+
+    @property
+    def ptr(self):
+        return self.data.ptr
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    def to_host(self) -> np.ndarray:
+        return self.data.to_host().reshape(self.K, self.C)
+
+    def copy_from(self, a):
+        self.data.copy_from(np.asarray(a).reshape(-1))
+
+    def copy(self):
+        o = CellArray.__new__(CellArray)
+        o.ctx, o.ni, o.K, o.C = self.ctx, self.ni, self.K, self.C
+        o.data = self.data.copy()
+        return o
+
+    def cell(self, *I):
+        """Host copy of the K values of cell I (0-based)."""
+        c = int(np.ravel_multi_index(I, self.ni, order="F"))
+        return self.to_host()[:, c]
+
+
+class DeviceAxis:
+    """An :class:`Axis` with device copies of x, dx and inv_dx."""
+
+    def __init__(self, ctx: Context, ax: Axis):
+        self.host = ax
+        self.x = DeviceArray.from_host(ctx, ax.x)
+        self.dx = DeviceArray.from_host(ctx, ax.dx)
+        self.inv_dx = DeviceArray.from_host(ctx, ax.inv_dx)
+        self.n = ax.n
+
+    def abi(self):
+        return _abi.jp_axis(self.x.ptr, self.dx.ptr, self.inv_dx.ptr, self.n)
+
+
+class Particles:
+    """Device particle container (``Particles{HIPBackend}``).
+
+    Fields follow the reference: ``coords`` (D CellArrays), ``index`` (Bool
+    CellArray), ``nxcell``, ``max_xcell``, ``min_xcell``, ``np``, grids ``xvi``,
+    ``xci``, ``xi_vel`` (host :class:`Axis` objects; ``di``/``_di`` are their
+    ``dx``/``inv_dx``).  ``seed`` and ``inject_counter`` key the Philox stream
+    that replaces Julia's ``rand()``.
+    """
+
+    def __init__(self, ctx, coords, index, nxcell, max_xcell, min_xcell, xvi, xci, xi_vel,
+                 seed=0):
+        self.ctx = ctx
+        self.coords = tuple(coords)
+        self.index = index
+        self.nxcell = int(nxcell)
+        self.max_xcell = int(max_xcell)
+        self.min_xcell = int(min_xcell)
+        self.ni = tuple(a.n - 1 for a in xvi)
+        self.np = self.max_xcell * int(np.prod(self.ni))
+        self.xvi = tuple(xvi)
+        self.xci = tuple(xci)
+        self.xi_vel = tuple(tuple(g) for g in xi_vel)
+        self.seed = int(seed)
+        self.inject_counter = 0
+        self._dxv = [DeviceAxis(ctx, a) for a in self.xvi]
+        self._dvel = [[DeviceAxis(ctx, a) for a in g] for g in self.xi_vel]
+        self._grid = self._make_grid()
+        self._velgrid = self._make_velgrid()
+
+    @property
+    def D(self):
+        return len(self.xvi)
+
+    @property
+    def C(self):
+        return int(np.prod(self.ni))
+
+    @property
+    def di(self):
+        return {"vertex": tuple(a.dx for a in self.xvi),
+                "center": tuple(a.dx for a in self.xci),
+                "velocity": tuple(tuple(a.dx for a in g) for g in self.xi_vel)}
+
+    def _make_grid(self):
+        g = _abi.jp_cellgrid()
+        g.ndim = self.D
+        g.max_xcell = self.max_xcell
+        for d in range(self.D):
+            g.ni[d] = self.ni[d]
+            g.xv[d] = self._dxv[d].abi()
+        return g
+
+    def _make_velgrid(self):
+        vg = _abi.jp_velgrid()
+        for c in range(self.D):
+            for d in range(self.D):
+                vg.ax[c][d] = self._dvel[c][d].abi()
+        return vg
+
+    def grid_ref(self):
+        return C.byref(self._grid)
+
+    def velgrid_ref(self):
+        return C.byref(self._velgrid)
+
+    def coord_ptrs(self):
+        return _abi.ptr_array([c.ptr for c in self.coords])
+
+    def active(self) -> int:
+        return int(self.index.to_host().sum())
+
+    def host_state(self):
+        """(coords list of flat arrays, index flat array), library layout."""
+        return ([c.data.flat() for c in self.coords], self.index.data.flat())
+
+
+def _split_grids(xi_vel):
+    D = len(xi_vel)
+    xvi = tuple(xi_vel[i][i] for i in range(D))
+    if D == 2:
+        xci = (xi_vel[1][0].inner(), xi_vel[0][1].inner())
+    else:
+        xci = (xi_vel[1][0].inner(), xi_vel[0][1].inner(), xi_vel[0][2].inner())
+    return xvi, xci
+
+
+def particles_from_host(ctx: Context, xi_vel, coords, index, nxcell, max_xcell, min_xcell,
+                        seed=0):
+    """Device Particles holding a given host state (library layout arrays),
+    e.g. a restart, or the same state an oracle run starts from."""
+    xi_vel = tuple(tuple(g) for g in xi_vel)
+    xvi, xci = _split_grids(xi_vel)
+    ni = tuple(a.n - 1 for a in xvi)
+    cs = []
+    for a in coords:
+        ca = CellArray(ctx, ni, max_xcell)
+        ca.copy_from(np.asarray(a, dtype=np.float64))
+        cs.append(ca)
+    idx = CellArray(ctx, ni, max_xcell, dtype=np.uint8)
+    idx.copy_from(np.asarray(index, dtype=np.uint8))
+    return Particles(ctx, cs, idx, nxcell, max_xcell, min_xcell, xvi, xci, xi_vel, seed=seed)
+
+
+def init_particles(ctx: Context, nxcell, max_xcell, min_xcell, *xi_vel, seed: int = 0):
+    """``init_particles(backend, nxcell, max_xcell, min_xcell, xi_vel...)``
+    (src/Particles/particles_utils.jl:30-166): nxcell rounded up to a multiple
+    of 2^D, K = max(nxcell, max_xcell), quadrant-stratified random fill."""
+    if len(xi_vel) == 1 and isinstance(xi_vel[0], tuple) and xi_vel and \
+            isinstance(xi_vel[0][0], tuple):
+        xi_vel = xi_vel[0]
+    if len(xi_vel) == 0:
+        raise ValueError("The velocity grid cannot be empty")
+    D = len(xi_vel)
+    if D not in (2, 3):
+        raise ValueError("The dimension of the problem must be either 2 or 3")
+    xi_vel = tuple(tuple(g) for g in xi_vel)
+    # xvi[i] = xi_vel[i][i]; xci = interior of the staggered axes (:56-74)
+    xvi, xci = _split_grids(xi_vel)
+    NQ = 4 if D == 2 else 8
+    npq = math.ceil(nxcell / NQ)
+    nxcell = npq * NQ
+    max_xcell = max(nxcell, max_xcell)
+    if max_xcell > 64:
+        raise ValueError("max_xcell must be at most 64 (one wavefront of slots)")
+    ni = tuple(a.n - 1 for a in xvi)
+    coords = [CellArray(ctx, ni, max_xcell) for _ in range(D)]
+    index = CellArray(ctx, ni, max_xcell, dtype=np.uint8)
+    p = Particles(ctx, coords, index, nxcell, max_xcell, min_xcell, xvi, xci, xi_vel, seed=seed)
+    ctx.check(ctx.lib.jp_init_particles(ctx.h, p.grid_ref(), p.coord_ptrs(), index.ptr, npq,
+                                        int(seed)), "init_particles")
+    return p
+
+
+def init_cell_arrays(particles: Particles, n: int, fill=0.0):
+    """``init_cell_arrays(particles, Val(n))`` (particles_utils.jl:294-301)."""
+    return tuple(CellArray(particles.ctx, particles.ni, particles.max_xcell, fill=fill)
+                 for _ in range(n))

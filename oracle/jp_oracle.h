@@ -1,0 +1,65 @@
+/*
+ * jp_oracle.h -- CPU FP64 oracle of JustPIC's particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a sequential C restatement of the
+ * reference algorithm (JuliaGeodynamics/JustPIC.jl v0.6.7) used as the parity
+ * checker for the HIP library and as the CPU baseline; nothing in the product
+ * path links, loads or calls it.  Arrays use the same layout as the library
+ * (element (cell c, slot k) at data[c + C*k], 0-based column-major cells) but
+ * live in HOST memory: every pointer inside jp_cellgrid / jp_velgrid passed to
+ * these functions is a host pointer.
+ *
+ * Pinning (what fixes each function independently of this restatement) is
+ * listed in the test files under tests/ that exercise it; functions without an
+ * independent pin are marked "parity unpinned" in their comment.
+ */
+#ifndef JP_ORACLE_H
+#define JP_ORACLE_H
+
+#include "justpic_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* primitives, exposed for known-answer tests */
+double jpo_lerp1(double t, double v0, double v1);
+double jpo_lerp2(const double v[4], const double t[2]);
+double jpo_lerp3(const double v[8], const double t[3]);
+int64_t jpo_bisect(double px, const double* x, int64_t n, int64_t seed1);
+void jpo_first_stage(int method, double alpha, double dt, int ndim,
+                     const double* v, const double* p, double* out);
+void jpo_second_stage(double alpha, double dt, int ndim, const double* v0,
+                      const double* v1, const double* p, double* out);
+double jpo_distance(int ndim, const double* a, const double* b);
+void jpo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double jpo_uniform(uint64_t seed, uint64_t cell, int slot, int dim,
+                   int purpose, uint64_t counter);
+int64_t jpo_cell_index(double x, const double* xv, int64_t n);
+
+/* hot path */
+int jpo_init_particles(const jp_cellgrid* g, double* const* coords,
+                       uint8_t* index, int np_quadrant, uint64_t seed);
+int jpo_advection(const jp_cellgrid* g, double* const* coords,
+                  const uint8_t* index, const double* const* V,
+                  const jp_velgrid* vg, int method, double alpha, double dt);
+int jpo_move_particles(const jp_cellgrid* g, double* const* coords,
+                       uint8_t* index, int nargs, double* const* args,
+                       jp_stats* stats);
+int jpo_particle2grid(const jp_cellgrid* g, double* F, const double* Fp,
+                      const double* const* coords, const uint8_t* index);
+int jpo_particle2grid_multi(const jp_cellgrid* g, int nfields, double* const* F,
+                            const double* const* Fp,
+                            const double* const* coords, const uint8_t* index);
+int jpo_grid2particle(const jp_cellgrid* g, double* Fp, const double* F,
+                      const double* const* coords, const uint8_t* index);
+int jpo_inject_particles(const jp_cellgrid* g, double* const* coords,
+                         uint8_t* index, int nargs, double* const* args,
+                         int min_xcell, uint64_t seed, uint64_t call_counter,
+                         jp_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,232 @@
+"""ctypes wrapper of the C oracle (oracle/jp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker or the reported CPU
+baseline, never as the product path.  Arrays are host numpy arrays in the
+library's layout (element (cell c, slot k) at [c + C*k]).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from justpic_hip._abi import (JP_EULER, JP_RK2, JP_RK4, jp_axis, jp_cellgrid,
+                              jp_stats, jp_velgrid, ptr_array)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjporacle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "jpo_lerp1": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "jpo_lerp2": (C.c_double, [P, P]),
+            "jpo_lerp3": (C.c_double, [P, P]),
+            "jpo_bisect": (C.c_int64, [C.c_double, P, C.c_int64, C.c_int64]),
+            "jpo_first_stage": (None, [C.c_int, C.c_double, C.c_double, C.c_int, P, P, P]),
+            "jpo_second_stage": (None, [C.c_double, C.c_double, C.c_int, P, P, P, P]),
+            "jpo_distance": (C.c_double, [C.c_int, P, P]),
+            "jpo_philox": (None, [P, P, P]),
+            "jpo_uniform": (C.c_double, [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                         C.c_uint64]),
+            "jpo_cell_index": (C.c_int64, [C.c_double, P, C.c_int64]),
+            "jpo_init_particles": (C.c_int, [C.POINTER(jp_cellgrid), P, P, C.c_int, C.c_uint64]),
+            "jpo_advection": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, C.POINTER(jp_velgrid),
+                                        C.c_int, C.c_double, C.c_double]),
+            "jpo_move_particles": (C.c_int, [C.POINTER(jp_cellgrid), P, P, C.c_int, P,
+                                             C.POINTER(jp_stats)]),
+            "jpo_particle2grid": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, P]),
+            "jpo_particle2grid_multi": (C.c_int, [C.POINTER(jp_cellgrid), C.c_int, P, P, P, P]),
+            "jpo_grid2particle": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, P]),
+            "jpo_inject_particles": (C.c_int, [C.POINTER(jp_cellgrid), P, P, C.c_int, P, C.c_int,
+                                               C.c_uint64, C.c_uint64, C.POINTER(jp_stats)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p).value
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise ValueError(f"oracle {what} returned {rc}")
+
+
+def host_axis(ax):
+    return jp_axis(_p(ax.x), _p(ax.dx), _p(ax.inv_dx), ax.n)
+
+
+# ---- primitives ---------------------------------------------------------
+
+def lerp(v, t):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    if t.size == 1:
+        return lib().jpo_lerp1(float(t[0]), float(v[0]), float(v[1]))
+    if t.size == 2:
+        return lib().jpo_lerp2(_p(v), _p(t))
+    return lib().jpo_lerp3(_p(v), _p(t))
+
+
+def bisect(px, x, seed1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().jpo_bisect(float(px), _p(x), x.size, int(seed1))
+
+
+def first_stage(method, alpha, dt, v, p):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty_like(p)
+    lib().jpo_first_stage(method, float(alpha), float(dt), p.size, _p(v), _p(p), _p(out))
+    return tuple(out)
+
+
+def second_stage(alpha, dt, v0, v1, p):
+    v0, v1, p = (np.ascontiguousarray(a, dtype=np.float64) for a in (v0, v1, p))
+    out = np.empty_like(p)
+    lib().jpo_second_stage(float(alpha), float(dt), p.size, _p(v0), _p(v1), _p(p), _p(out))
+    return tuple(out)
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().jpo_philox(_p(c), _p(k), _p(out))
+    return out
+
+
+def uniform(seed, cell, slot, dim, purpose, counter=0):
+    return lib().jpo_uniform(seed, cell, slot, dim, purpose, counter)
+
+
+def cell_index(x, xv):
+    xv = np.ascontiguousarray(xv, dtype=np.float64)
+    return lib().jpo_cell_index(float(x), _p(xv), xv.size)
+
+
+# ---- particle state -----------------------------------------------------
+
+class HostParticles:
+    """Host copy of a particle container in the library layout."""
+
+    def __init__(self, xv_axes, K, xi_vel=None):
+        self.D = len(xv_axes)
+        self.xv = tuple(xv_axes)
+        self.ni = tuple(a.n - 1 for a in xv_axes)
+        self.K = int(K)
+        self.C = int(np.prod(self.ni))
+        self.S = self.C * self.K
+        self.xi_vel = xi_vel
+        self.coords = [np.full(self.S, np.nan) for _ in range(self.D)]
+        self.index = np.zeros(self.S, dtype=np.uint8)
+        self._grid = self.cellgrid()
+
+    def cellgrid(self):
+        g = jp_cellgrid()
+        g.ndim = self.D
+        g.max_xcell = self.K
+        for d in range(self.D):
+            g.ni[d] = self.ni[d]
+            g.xv[d] = host_axis(self.xv[d])
+        return g
+
+    def velgrid(self):
+        vg = jp_velgrid()
+        for c in range(self.D):
+            for d in range(self.D):
+                vg.ax[c][d] = host_axis(self.xi_vel[c][d])
+        return vg
+
+    def copy(self):
+        o = HostParticles.__new__(HostParticles)
+        o.__dict__.update(self.__dict__)
+        o.coords = [a.copy() for a in self.coords]
+        o.index = self.index.copy()
+        o._grid = o.cellgrid()
+        return o
+
+    def coord_ptrs(self):
+        return ptr_array([_p(a) for a in self.coords])
+
+
+def init_particles(hp: HostParticles, np_quadrant, seed):
+    _chk(lib().jpo_init_particles(C.byref(hp._grid), hp.coord_ptrs(), _p(hp.index),
+                                  int(np_quadrant), int(seed)), "init")
+
+
+def advection(hp: HostParticles, V, method, alpha, dt):
+    V = [np.ascontiguousarray(v.ravel(order="F"), dtype=np.float64) for v in V]
+    vg = hp.velgrid()
+    _chk(lib().jpo_advection(C.byref(hp._grid), hp.coord_ptrs(), _p(hp.index),
+                             ptr_array([_p(v) for v in V]), C.byref(vg), int(method),
+                             float(alpha), float(dt)), "advection")
+
+
+def move_particles(hp: HostParticles, args=()):
+    st = jp_stats()
+    _chk(lib().jpo_move_particles(C.byref(hp._grid), hp.coord_ptrs(), _p(hp.index), len(args),
+                                  ptr_array([_p(a) for a in args]), C.byref(st)), "move")
+    return st.as_dict()
+
+
+def particle2grid(hp: HostParticles, Fp):
+    nn = tuple(n + 1 for n in hp.ni)
+    F = np.zeros(int(np.prod(nn)))
+    _chk(lib().jpo_particle2grid(C.byref(hp._grid), _p(F), _p(Fp), hp.coord_ptrs(),
+                                 _p(hp.index)), "p2g")
+    return F
+
+
+def particle2grid_multi(hp: HostParticles, Fps):
+    nn = tuple(n + 1 for n in hp.ni)
+    Fs = [np.zeros(int(np.prod(nn))) for _ in Fps]
+    _chk(lib().jpo_particle2grid_multi(C.byref(hp._grid), len(Fps),
+                                       ptr_array([_p(f) for f in Fs]),
+                                       ptr_array([_p(f) for f in Fps]), hp.coord_ptrs(),
+                                       _p(hp.index)), "p2g multi")
+    return Fs
+
+
+def grid2particle(hp: HostParticles, F, Fp=None):
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).ravel(order="F"))
+    if Fp is None:
+        Fp = np.zeros(hp.S)
+    _chk(lib().jpo_grid2particle(C.byref(hp._grid), _p(Fp), _p(F), hp.coord_ptrs(),
+                                 _p(hp.index)), "g2p")
+    return Fp
+
+
+def inject_particles(hp: HostParticles, args, min_xcell, seed, counter):
+    st = jp_stats()
+    _chk(lib().jpo_inject_particles(C.byref(hp._grid), hp.coord_ptrs(), _p(hp.index), len(args),
+                                    ptr_array([_p(a) for a in args]), int(min_xcell), int(seed),
+                                    int(counter), C.byref(st)), "inject")
+    return st.as_dict()
+
+
+__all__ = ["JP_EULER", "JP_RK2", "JP_RK4", "HostParticles", "init_particles", "advection",
+           "move_particles", "particle2grid", "particle2grid_multi", "grid2particle",
+           "inject_particles", "lerp", "bisect", "first_stage", "second_stage", "philox",
+           "uniform", "cell_index", "build", "lib"]
