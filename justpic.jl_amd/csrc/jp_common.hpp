@@ -45,6 +45,10 @@ struct CPtr3 {
 struct ArgPtrs {
     double* p[JP_MAX_ARGS];
 };
+// 0-based colour offsets of one phase
+struct Off3 {
+    int o[3];
+};
 
 // 0-based cell tuple <-> column-major linear index
 template <int D>
@@ -98,6 +102,22 @@ __device__ inline int64_t bisect_exact(double p, const double* __restrict__ x, i
 // ties (p equal to a grid value) depend on the search path, and those run the
 // reference loop itself.  Candidates: the seed, its neighbours, and the index a
 // uniform spacing would give.
+// Out-of-line part (rare: displacement beyond one cell, refined axes, ties).
+__device__ __noinline__ int64_t find_cell_slow(double p, const double* __restrict__ x, int64_t n,
+                                               int64_t s) {
+    const int64_t last = n - 2;
+    double x0 = x[0], xn = x[n - 1];
+    double fg = (p - x0) / (xn - x0) * (double)(n - 1);
+    int64_t g = (int64_t)fg;
+    g = g < 0 ? 0 : (g > last ? last : g);
+    for (int o = 0; o < 3; ++o) {
+        int64_t h = g + (o == 0 ? 0 : (o == 1 ? -1 : 1));
+        if (h >= 0 && h <= last && x[h] < p && p < x[h + 1]) return h;
+    }
+    int64_t r = bisect_exact(p, x, n, s);
+    return r < 0 ? 0 : r;
+}
+
 __device__ inline int64_t find_cell(double p, const Axis& a, int64_t seed) {
     const double* __restrict__ x = a.x;
     const int64_t last = a.n - 2;
@@ -113,18 +133,97 @@ __device__ inline int64_t find_cell(double p, const Axis& a, int64_t seed) {
         double xp = x[s + 2];
         if (xs1 < p && p < xp) return s + 1;
     }
-    double x0 = x[0], xn = x[a.n - 1];
-    double fg = (p - x0) / (xn - x0) * (double)(a.n - 1);
-    int64_t g = (int64_t)fg;
-    g = g < 0 ? 0 : (g > last ? last : g);
-#pragma unroll
-    for (int o = 0; o < 3; ++o) {
-        int64_t h = g + (o == 0 ? 0 : (o == 1 ? -1 : 1));
-        if (h >= 0 && h <= last && x[h] < p && p < x[h + 1]) return h;
-    }
-    int64_t r = bisect_exact(p, x, a.n, s);
-    return r < 0 ? 0 : r;
+    return find_cell_slow(p, x, a.n, s);
 }
+
+// Row mapping of cell-parallel kernels: each wavefront owns 64 consecutive
+// cells along x of one grid row (fixed j [,k]); blocks hold kBlock/64 rows.
+template <int D>
+__host__ __device__ inline int64_t row_count(const Grid<D>& g) {
+    int64_t r = (g.ni[0] + 63) / 64;
+    for (int d = 1; d < D; ++d) r *= g.ni[d];
+    return r;
+}
+template <int D>
+inline int64_t row_blocks(const Grid<D>& g) {
+    return (row_count<D>(g) + kBlock / 64 - 1) / (kBlock / 64);
+}
+template <int D>
+__device__ inline bool row_cell(const Grid<D>& g, int64_t& c, int64_t (&I)[D]) {
+    const int lane = threadIdx.x & 63;
+    // wave-uniform row index (decoded with scalar arithmetic)
+    int64_t row = (int64_t)blockIdx.x * (kBlock / 64) +
+                  __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (row >= row_count<D>(g)) return false;
+    const int64_t nb = (g.ni[0] + 63) / 64;
+    I[0] = (row % nb) * 64 + lane;
+    row /= nb;
+#pragma unroll
+    for (int d = 1; d < D; ++d) {
+        I[d] = row % g.ni[d];
+        row /= g.ni[d];
+    }
+    if (I[0] >= g.ni[0]) return false;
+    c = lin<D>(g.ni, I);
+    return true;
+}
+
+// Active slots of cell c as a bitmask: the K mask bytes are loaded in
+// unrolled chunks of 8 independent loads (coalesced across neighbouring
+// cells of a wavefront in the cell-plane layout).
+__device__ inline uint64_t active_slots(const uint8_t* __restrict__ index, int64_t c, int64_t C,
+                                        int K) {
+    uint64_t bits = 0;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        uint8_t m[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) m[v] = (k0 + v < K) ? index[c + C * (k0 + v)] : (uint8_t)0;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) bits |= (uint64_t)(m[v] != 0) << (k0 + v);
+    }
+    return bits;
+}
+
+// Iterates the set bits of `bits` in ascending order, loading NV values per
+// slot (from planes p[0..NV-1], element c + C*k) two slots ahead of use.
+template <int NV>
+struct SlotStream {
+    const double* const* p;
+    int64_t c, C;
+    uint64_t bits;
+    int k1, k2;
+    double v1[NV], v2[NV];
+    __device__ inline void load(int k, double (&v)[NV]) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = p[i][c + C * k];
+    }
+    __device__ inline int pop() {
+        if (!bits) return -1;
+        int k = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        return k;
+    }
+    __device__ inline SlotStream(const double* const* planes, int64_t c_, int64_t C_, uint64_t b)
+        : p(planes), c(c_), C(C_), bits(b) {
+        k1 = pop();
+        if (k1 >= 0) load(k1, v1);
+        k2 = pop();
+        if (k2 >= 0) load(k2, v2);
+    }
+    // next slot and its values; -1 when done
+    __device__ inline int next(double (&out)[NV]) {
+        const int k = k1;
+        if (k < 0) return -1;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) out[i] = v1[i];
+        k1 = k2;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v1[i] = v2[i];
+        k2 = pop();
+        if (k2 >= 0) load(k2, v2);
+        return k;
+    }
+};
 
 // ---- Philox4x32-10 (Salmon et al., SC11) --------------------------------
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

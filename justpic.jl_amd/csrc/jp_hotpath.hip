@@ -1,6 +1,7 @@
 // jp_hotpath.hip -- FP64 drop-in kernels for JustPIC's particle hot path on
-// gfx950: init, advection (Euler/RK2/RK4), move, particle2grid, grid2particle
-// and injection, 2D and 3D, on the B = 0 cell-plane layout data[c + C*k].
+// gfx950: init, particle2grid, grid2particle and injection, 2D and 3D, on the
+// B = 0 cell-plane layout data[c + C*k] (advection: jp_advect.hip, move:
+// jp_move.hip).
 #include "jp_ctx.hpp"
 
 #include <cmath>
@@ -8,11 +9,6 @@
 using namespace jp;
 
 namespace {
-
-template <int D>
-struct VPtrs {
-    const double* v[D];
-};
 
 __device__ inline double dnan() { return __builtin_nan(""); }
 
@@ -54,267 +50,6 @@ __global__ void __launch_bounds__(kBlock) k_init(Grid<D> g, Ptr3 coords, uint8_t
     }
 }
 
-// ============================================================ advection
-// interp_velocity2particle (src/Particles/Advection/advection.jl:93-148) for
-// all components: +Inf outside the component's grid (check_local_limits,
-// src/Advection/advection.jl:83-91), else bisection seeded at the particle's
-// cell, t = (p - x_k)*inv(dx_k), n-linear lerp of the corner values.
-template <int D>
-__device__ inline void interp_vel(const Vel<D>& vg, const VPtrs<D>& V, const double (&p)[D],
-                                  const int64_t (&I)[D], double (&v)[D]) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        const Axis* ax = vg.ax[c];
-        bool in = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) in = in && (ax[d].x[0] <= p[d] && p[d] <= ax[d].x[ax[d].n - 1]);
-        if (!in) {
-            v[c] = __builtin_inf();
-            continue;
-        }
-        int64_t k[D];
-        double t[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            k[d] = find_cell(p[d], ax[d], I[d]);
-            t[d] = (p[d] - ax[d].x[k[d]]) * ax[d].idx[k[d]];
-        }
-        const double* __restrict__ F = V.v[c];
-        const int64_t n0 = ax[0].n;
-        if constexpr (D == 2) {
-            int64_t b = k[0] + n0 * k[1];
-            double F00 = F[b], F10 = F[b + 1], F01 = F[b + n0], F11 = F[b + n0 + 1];
-            v[c] = lerp1(t[1], lerp1(t[0], F00, F10), lerp1(t[0], F01, F11));
-        } else {
-            const int64_t n1 = ax[1].n;
-            int64_t b = k[0] + n0 * (k[1] + n1 * k[2]);
-            int64_t s = n0 * n1;
-            double F000 = F[b], F100 = F[b + 1], F010 = F[b + n0], F110 = F[b + n0 + 1];
-            double F001 = F[b + s], F101 = F[b + s + 1], F011 = F[b + s + n0],
-                   F111 = F[b + s + n0 + 1];
-            double lo = lerp1(t[1], lerp1(t[0], F000, F100), lerp1(t[0], F010, F110));
-            double hi = lerp1(t[1], lerp1(t[0], F001, F101), lerp1(t[0], F011, F111));
-            v[c] = lerp1(t[2], lo, hi);
-        }
-    }
-}
-
-struct RKParams {
-    double dt;
-    double adt;  // (s*alpha)*dt
-    int half;    // alpha == 0.5
-    double w0;   // muladd(-0.5, inv(alpha), 1)
-    double w1;   // 0.5*inv(alpha)
-};
-
-// advect_particle: Euler (Euler.jl), RK2 (RK2.jl), RK4 (RK4.jl)
-template <int D, int M>
-__device__ inline void advect_one(const Vel<D>& vg, const VPtrs<D>& V, const RKParams& P,
-                                  const double (&p0)[D], const int64_t (&I)[D], double (&out)[D]) {
-    double v0[D];
-    interp_vel<D>(vg, V, p0, I, v0);
-    if constexpr (M == JP_EULER) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v0[d], p0[d]);
-    } else if constexpr (M == JP_RK2) {
-        double p1[D], v1[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) p1[d] = fma(P.adt, v0[d], p0[d]);
-        interp_vel<D>(vg, V, p1, I, v1);
-        if (P.half) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v1[d], p0[d]);
-        } else {
-#pragma unroll
-            for (int d = 0; d < D; ++d) out[d] = fma(P.dt, fma(P.w1, v1[d], P.w0 * v0[d]), p0[d]);
-        }
-    } else {
-        double q[D], k2[D], k3[D], k4[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * v0[d]) / 2.0;
-        interp_vel<D>(vg, V, q, I, k2);
-#pragma unroll
-        for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * k2[d]) / 2.0;
-        interp_vel<D>(vg, V, q, I, k3);
-#pragma unroll
-        for (int d = 0; d < D; ++d) q[d] = p0[d] + P.dt * k3[d];
-        interp_vel<D>(vg, V, q, I, k4);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            double sum = ((v0[d] + 2.0 * k2[d]) + 2.0 * k3[d]) + k4[d];
-            out[d] = p0[d] + (P.dt * sum) / 6.0;
-        }
-    }
-}
-
-// advection_kernel! (advection.jl:66-91): one thread per slot; blockIdx.y is
-// the slot plane, so a wavefront reads 64 consecutive cells of one plane.
-template <int D, int M>
-__global__ void __launch_bounds__(kBlock) k_advect(Grid<D> g, Vel<D> vg, VPtrs<D> V, Ptr3 coords,
-                                                   const uint8_t* __restrict__ index, RKParams P) {
-    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= g.C) return;
-    int64_t e = c + g.C * (int64_t)blockIdx.y;
-    if (!index[e]) return;
-    int64_t I[D];
-    unlin<D>(g.ni, c, I);
-    double p0[D], p[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) p0[d] = coords.p[d][e];
-    advect_one<D, M>(vg, V, P, p0, I, p);
-#pragma unroll
-    for (int d = 0; d < D; ++d) coords.p[d][e] = p[d];
-}
-
-// ============================================================ move
-template <int D>
-__device__ inline bool in_cell(const Grid<D>& g, const int64_t (&I)[D], const double (&p)[D]) {
-    bool b = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        double x = g.xv[d].x[I[d]], h = g.xv[d].dx[I[d]];
-        b = b & (x < p[d] && p[d] < x + h);  // isincell, src/Particles/utils.jl:15
-    }
-    return b;
-}
-
-template <int D>
-__device__ inline bool in_domain(const Grid<D>& g, const double (&p)[D]) {
-#pragma unroll
-    for (int d = 0; d < D; ++d)  // indomain, src/Particles/move_safe.jl:199-206
-        if (!(g.xv[d].x[0] < p[d] && p[d] < g.xv[d].x[g.xv[d].n - 1])) return false;
-    return true;
-}
-
-// Detects movers whose destination is not a neighbour (CFL > 1).  Their
-// presence makes the colour phases' neighbourhoods overlap, so the phases
-// then run their cells one after another in a single thread.
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_move_far(Grid<D> g, CPtr3 coords,
-                                                     const uint8_t* __restrict__ index, int* flag) {
-    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= g.C) return;
-    int64_t e = c + g.C * (int64_t)blockIdx.y;
-    if (!index[e]) return;
-    int64_t I[D];
-    unlin<D>(g.ni, c, I);
-    double p[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-    if (in_cell<D>(g, I, p) || !in_domain<D>(g, p)) return;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        int64_t nc = find_cell(p[d], g.xv[d], I[d]);
-        if (nc - I[d] > 1 || I[d] - nc > 1) {
-            atomicOr(flag, 1);
-            return;
-        }
-    }
-}
-
-// move_kernel! (src/Particles/move_safe.jl:72-125) for one source cell.
-template <int D>
-__device__ void move_cell(const Grid<D>& g, const Ptr3& coords, uint8_t* __restrict__ index,
-                          int nargs, const ArgPtrs& args, const int64_t (&I)[D], int64_t (&cnt)[4]) {
-    const int64_t C = g.C;
-    const int K = g.K;
-    const int64_t c = lin<D>(g.ni, I);
-    int sp = 0;  // starting_point, 0-based
-    for (int ip = 0; ip < K; ++ip) {
-        const int64_t e = c + C * ip;
-        if (!index[e]) continue;
-        double p[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-        if (in_cell<D>(g, I, p)) continue;
-        if (!in_domain<D>(g, p)) {
-            index[e] = 0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
-            for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-            cnt[1]++;
-            continue;
-        }
-        int64_t nc[D];
-        bool far = false;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            nc[d] = find_cell(p[d], g.xv[d], I[d]);
-            far = far || (nc[d] - I[d] > 1 || I[d] - nc[d] > 1);
-        }
-        if (far) cnt[3]++;
-        double cached[JP_MAX_ARGS];
-        for (int a = 0; a < nargs; ++a) cached[a] = args.p[a][e];
-        index[e] = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
-        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-        const int64_t cn = lin<D>(g.ni, nc);
-        int fi = -1;
-        for (int i = sp; i < K; ++i)
-            if (!index[cn + C * i]) {
-                fi = i;
-                break;
-            }
-        if (fi < 0) {
-            cnt[2]++;
-            continue;
-        }
-        sp = fi;
-        const int64_t f = cn + C * fi;
-        index[f] = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][f] = p[d];
-        for (int a = 0; a < nargs; ++a) args.p[a][f] = cached[a];
-        cnt[0]++;
-    }
-}
-
-struct Off3 {
-    int o[3];
-};
-
-// move_particles_ps! (src/Particles/move_safe.jl:51-62): one thread per cell
-// of the colour (cells 3*T + offset).
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_move_phase(Grid<D> g, Ptr3 coords, uint8_t* index,
-                                                       int nargs, ArgPtrs args, Off3 off,
-                                                       const int* __restrict__ flag, jp_stats* st) {
-    int64_t ncol[D];
-    int64_t total = 1;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        ncol[d] = (g.ni[d] + 2) / 3;
-        total *= ncol[d];
-    }
-    int64_t cnt[4] = {0, 0, 0, 0};
-    int64_t t0, t1;
-    if (*flag) {
-        if (blockIdx.x != 0 || threadIdx.x != 0) return;
-        t0 = 0;
-        t1 = total;
-    } else {
-        t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-        if (t0 >= total) return;
-        t1 = t0 + 1;
-    }
-    for (int64_t t = t0; t < t1; ++t) {
-        int64_t r = t, I[D];
-        bool ok = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            I[d] = 3 * (r % ncol[d]) + off.o[d];
-            r /= ncol[d];
-            ok = ok && I[d] < g.ni[d];
-        }
-        if (ok) move_cell<D>(g, coords, index, nargs, args, I, cnt);
-    }
-    stat_add(st, ST_MOVED, cnt[0]);
-    stat_add(st, ST_DELETED, cnt[1]);
-    stat_add(st, ST_DROPPED, cnt[2]);
-    stat_add(st, ST_FAR, cnt[3]);
-}
-
 // ============================================================ particle2grid
 // _particle2grid! (src/Interpolations/particle_to_grid.jl:37-199): one thread
 // per node, exact reference accumulation order (cells with the last dim
@@ -351,6 +86,8 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Grid<D> g, int nf_rt, ArgPtrs F,
     for (int f = 0; f < nf; ++f) wF[f] = 0.0;
     const int64_t C = g.C;
     constexpr int NC = 1 << D;
+    // the (up to) 2^D cells around the node, last dim outermost
+    int64_t cell[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         int64_t I[D];
@@ -360,25 +97,59 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Grid<D> g, int nf_rt, ArgPtrs F,
             I[d] = N[d] + (((q >> d) & 1) ? 0 : -1);
             ok = ok && (I[d] >= 0 && I[d] < g.ni[d]);
         }
-        if (!ok) continue;
-        const int64_t c = lin<D>(g.ni, I);
-        for (int ip = 0; ip < g.K; ++ip) {
-            const int64_t e = c + C * ip;
-            if (!index[e]) continue;
-            double s;
-            {
-                double a = xn[0] - coords.p[0][e];
-                s = a * a;
+        cell[q] = ok ? lin<D>(g.ni, I) : -1;
+    }
+    // their mask bytes, all cells' chunk loads in flight together
+    uint64_t bits[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) bits[q] = 0;
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+        uint8_t m[NC][8];
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                m[q][v] = (cell[q] >= 0 && k0 + v < g.K) ? index[cell[q] + C * (k0 + v)] : (uint8_t)0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) bits[q] |= (uint64_t)(m[q][v] != 0) << (k0 + v);
+    }
+    // reference order (cells, then slots ascending).  All lanes of a
+    // wavefront walk the same slot plane at the same time, so every load is
+    // coalesced across the wavefront's neighbouring cells; a chunk of slots is
+    // loaded (predicated on the mask bit) before it is accumulated.
+    constexpr int B = D == 2 ? 16 : 8;
+#pragma unroll 1
+    for (int q = 0; q < NC; ++q) {
+        const int64_t c = cell[q] < 0 ? 0 : cell[q];
+        const uint64_t bq = bits[q];
+        for (int k0 = 0; k0 < g.K; k0 += B) {
+            double pv[B][D + 1];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const bool on = (bq >> (k0 + j)) & 1;  // bits above K are 0
+                const int64_t e = c + C * (k0 + j);
+#pragma unroll
+                for (int d = 0; d < D; ++d) pv[j][d] = on ? coords.p[d][e] : 0.0;
+                pv[j][D] = on ? Fp.p[0][e] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (!((bq >> (k0 + j)) & 1)) continue;
+                double a = xn[0] - pv[j][0];
+                double s = a * a;
 #pragma unroll
                 for (int d = 1; d < D; ++d) {
-                    double b = xn[d] - coords.p[d][e];
+                    double b = xn[d] - pv[j][d];
                     s = s + b * b;
                 }
+                const double dist = sqrt(s);
+                const double wi = 1.0 / (dist * dist);
+                w += wi;
+                wF[0] = fma(wi, pv[j][D], wF[0]);
+                for (int f = 1; f < nf; ++f) wF[f] = fma(wi, Fp.p[f][c + C * (k0 + j)], wF[f]);
             }
-            double dist = sqrt(s);
-            double wi = 1.0 / (dist * dist);
-            w += wi;
-            for (int f = 0; f < nf; ++f) wF[f] = fma(wi, Fp.p[f][e], wF[f]);
         }
     }
     if (mode == 0) {
@@ -395,12 +166,10 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_g2p(Grid<D> g, double* __restrict__ Fp,
                                                 const double* __restrict__ F, CPtr3 coords,
                                                 const uint8_t* __restrict__ index) {
-    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= g.C) return;
-    int64_t e = c + g.C * (int64_t)blockIdx.y;
+    int64_t c, I[D];
+    if (!row_cell<D>(g, c, I)) return;
+    const int64_t e = c + g.C * (int64_t)blockIdx.y;
     if (!index[e]) return;
-    int64_t I[D];
-    unlin<D>(g.ni, c, I);
     double t[D];
 #pragma unroll
     for (int d = 0; d < D; ++d)
@@ -421,151 +190,225 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Grid<D> g, double* __restrict__ 
 }
 
 // ============================================================ injection
-// index_min_distance (src/Particles/injection.jl:330-393)
+// The first quadrant decides whether a cell injects at all: the reference
+// breaks out of the quadrant loop as soon as one quadrant holds min_q
+// particles (src/Particles/injection.jl:100), and injection into a cell never
+// changes the particles of other cells.  One coalesced pass over all cells
+// therefore lists the only cells that can inject, per colour; the colour
+// phases then run the reference loop on those cells only.
 template <int D>
-__device__ int64_t nearest(const Grid<D>& g, const Ptr3& coords, const uint8_t* index,
-                           const double (&pn)[D], int cur, const int64_t (&Ic)[D]) {
-    int64_t best = -1;
-    double dmin = __builtin_inf();
-    constexpr int NB = D == 2 ? 9 : 27;
-    for (int q = 0; q < NB; ++q) {
-        int64_t I[D];
-        int r = q;
-        bool ok = true, same = true;
+__global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
+                                                        const uint8_t* __restrict__ index,
+                                                        int min_q, int32_t* __restrict__ lists,
+                                                        int32_t* __restrict__ counts,
+                                                        int64_t cap) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    int64_t I[D];
+    unlin<D>(g.ni, c, I);
+    double lo[D], hq[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            I[d] = Ic[d] + (r % 3) - 1;
-            r /= 3;
-            ok = ok && (I[d] >= 0 && I[d] < g.ni[d]);
-            same = same && (I[d] == Ic[d]);
+    for (int d = 0; d < D; ++d) {
+        lo[d] = g.xv[d].x[I[d]];
+        hq[d] = g.xv[d].dx[I[d]] / 2.0;
+    }
+    // quadrant c11: vertex + dq*0 (src/Particles/injection.jl:442-448)
+    double q0[D], q1[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        q0[d] = lo[d] + hq[d] * 0.0;
+        q1[d] = q0[d] + hq[d];
+    }
+    int num = 0;
+    const uint64_t act = active_slots(index, c, g.C, g.K);
+    constexpr int B = 16;  // slots per chunk; lanes share the slot plane (coalesced)
+    for (int k0 = 0; k0 < g.K; k0 += B) {
+        double pb[B][D];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const bool on = (act >> (k0 + j)) & 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + g.C * (k0 + j)] : 0.0;
         }
-        if (!ok) continue;
-        const int64_t c = lin<D>(g.ni, I);
-        for (int ip = 0; ip < g.K; ++ip) {
-            if (same && ip == cur) continue;
-            const int64_t e = c + g.C * ip;
-            if (!index[e]) continue;
-            double p[D];
-            bool hasnan = false;
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                p[d] = coords.p[d][e];
-                hasnan = hasnan || (p[d] != p[d]);
-            }
-            if (D == 2 && hasnan) continue;
-            double a = p[0] - pn[0];
-            double s = a * a;
+        for (int j = 0; j < B; ++j) {
+            bool inside = (act >> (k0 + j)) & 1;
 #pragma unroll
-            for (int d = 1; d < D; ++d) {
-                double b = p[d] - pn[d];
-                s = s + b * b;
-            }
-            double dd = sqrt(s);
-            if (dd < dmin) {
-                best = e;
-                dmin = dd;
-            }
+            for (int d = 0; d < D; ++d) inside = inside & (q0[d] < pb[j][d] && pb[j][d] < q1[d]);
+            num += inside ? 1 : 0;
         }
     }
-    return best;
+    if (num >= min_q) return;
+    int col = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) col = col * 2 + (int)(I[d] & 1);
+    const int32_t slot = atomicAdd(&counts[col], 1);
+    lists[col * cap + slot] = (int32_t)c;
 }
 
-// _inject_particles! (src/Particles/injection.jl:68-131) for one cell
+// _inject_particles! (src/Particles/injection.jl:68-131) with one wavefront per
+// listed cell: lane k owns slot k of the cell (K <= 64).  Quadrant counts are
+// ballots; injected particles fill the free slots in ascending order, and the
+// nearest-neighbour search of index_min_distance (:330-393) evaluates the
+// 3^D x K candidates lane-parallel, keeping the reference's first-occurrence
+// tie rule (strict '<' in loop order: neighbour cell, then slot) through a
+// (distance, order) argmin.
 template <int D>
-__device__ void inject_cell(const Grid<D>& g, const Ptr3& coords, uint8_t* index, int nargs,
-                            const ArgPtrs& args, int min_q, uint64_t seed, uint64_t counter,
-                            const int64_t (&I)[D], int64_t (&cnt)[2]) {
-    const int64_t c = lin<D>(g.ni, I);
+__global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, uint8_t* index,
+                                                        int nargs, ArgPtrs args, int min_q,
+                                                        uint64_t seed, uint64_t counter,
+                                                        const int32_t* __restrict__ list,
+                                                        const int32_t* __restrict__ count,
+                                                        jp_stats* st) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6);
+    if (w >= *count) return;  // uniform across the wavefront
+    const int64_t c = list[w];
+    const int64_t C = g.C;
+    const int K = g.K;
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    int64_t I[D];
+    unlin<D>(g.ni, c, I);
     constexpr int NQ = D == 2 ? 4 : 8;
+    constexpr int NB = D == 2 ? 9 : 27;
+    constexpr int CENTER = (NB - 1) / 2;
     double xv[D], dq[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         xv[d] = g.xv[d].x[I[d]];
         dq[d] = g.xv[d].dx[I[d]] / 2.0;
     }
+    // this cell's slots, one per lane, kept current in registers
+    const bool own = lane < K;
+    const int64_t e = c + C * lane;
+    bool act = own && index[e];
+    double p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) p[d] = act ? coords.p[d][e] : 0.0;
+    int64_t n_inj = 0, n_orph = 0;
     for (int q = 0; q < NQ; ++q) {
         double vert[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) vert[d] = xv[d] + dq[d] * (double)((q >> d) & 1);
-        int num = 0;
-        for (int ip = 0; ip < g.K; ++ip) {
-            const int64_t e = c + g.C * ip;
-            if (!index[e]) continue;
-            bool inside = true;
+        bool inside = act;
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                double p = coords.p[d][e];
-                inside = inside & (vert[d] < p && p < vert[d] + dq[d]);
-            }
-            num += inside ? 1 : 0;
-        }
+        for (int d = 0; d < D; ++d) inside = inside & (vert[d] < p[d] && p[d] < vert[d] + dq[d]);
+        int num = __popcll(__ballot(inside));
         if (num >= min_q) break;
-        for (int ip = 0; ip < g.K; ++ip) {
-            const int64_t e = c + g.C * ip;
-            if (index[e]) continue;
+        uint64_t freeS = ~__ballot(act) & full;
+        bool done = false;
+        while (freeS) {
+            const int ip = __builtin_ctzll(freeS);
+            freeS &= freeS - 1;
             ++num;
             double pn[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                double r = uniform53(seed, (uint64_t)c, ip, d, 2, counter);
+                const double r = uniform53(seed, (uint64_t)c, ip, d, 2, counter);
                 pn[d] = vert[d] + dq[d] * fma(0.95, r, 0.05);
             }
+            const int64_t enew = c + C * ip;
+            if (lane == ip) {
+                act = true;
 #pragma unroll
-            for (int d = 0; d < D; ++d) coords.p[d][e] = pn[d];
-            index[e] = 1;
-            int64_t src = nearest<D>(g, coords, index, pn, ip, I);
-            if (src >= 0) {
-                for (int a = 0; a < nargs; ++a) args.p[a][e] = args.p[a][src];
-            } else {
-                cnt[1]++;
+                for (int d = 0; d < D; ++d) {
+                    p[d] = pn[d];
+                    coords.p[d][enew] = pn[d];
+                }
+                index[enew] = 1;
             }
-            cnt[0]++;
-            if (num >= min_q) break;
+            // index_min_distance over the 3^D neighbourhood
+            double best = __builtin_inf();
+            int border = 0x7fffffff;
+            constexpr int NBB = 9;  // neighbour cells per batch: masks, then coordinates
+#pragma unroll
+            for (int nb0 = 0; nb0 < NB; nb0 += NBB) {
+                bool a[NBB];
+                int64_t e2[NBB];
+#pragma unroll
+                for (int u = 0; u < NBB; ++u) {
+                    const int nb = nb0 + u;
+                    int64_t J[D];
+                    bool ok = own;
+                    int r = nb;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        J[d] = I[d] + (r % 3) - 1;
+                        r /= 3;
+                        ok = ok && J[d] >= 0 && J[d] < g.ni[d];
+                    }
+                    e2[u] = ok ? lin<D>(g.ni, J) + C * lane : 0;
+                    if (nb == CENTER)
+                        a[u] = act && lane != ip;
+                    else
+                        a[u] = ok && index[e2[u]] != 0;
+                }
+                double pq[NBB][D];
+#pragma unroll
+                for (int u = 0; u < NBB; ++u)
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+                        pq[u][d] = (nb0 + u == CENTER) ? p[d] : (a[u] ? coords.p[d][e2[u]] : 0.0);
+#pragma unroll
+                for (int u = 0; u < NBB; ++u) {
+                    if (!a[u]) continue;
+                    bool hasnan = false;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) hasnan = hasnan || (pq[u][d] != pq[u][d]);
+                    if (D == 2 && hasnan) continue;
+                    double du = pq[u][0] - pn[0];
+                    double s = du * du;
+#pragma unroll
+                    for (int d = 1; d < D; ++d) {
+                        double dv = pq[u][d] - pn[d];
+                        s = s + dv * dv;
+                    }
+                    const double dd = sqrt(s);
+                    if (dd < best) {
+                        best = dd;
+                        border = (nb0 + u) * 64 + lane;
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) {
+                const double ob = __shfl_xor(best, m, 64);
+                const int oo = __shfl_xor(border, m, 64);
+                if (ob < best || (ob == best && oo < border)) {
+                    best = ob;
+                    border = oo;
+                }
+            }
+            if (border == 0x7fffffff) {
+                ++n_orph;
+            } else if (lane == ip) {
+                int64_t J[D];
+                int r = border >> 6;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    J[d] = I[d] + (r % 3) - 1;
+                    r /= 3;
+                }
+                const int64_t esrc = lin<D>(g.ni, J) + C * (border & 63);
+                for (int a = 0; a < nargs; ++a) args.p[a][enew] = args.p[a][esrc];
+            }
+            // later searches of this wavefront read what lane ip just wrote
+            __threadfence_block();
+            ++n_inj;
+            if (num >= min_q) {
+                done = true;
+                break;
+            }
         }
+        (void)done;
     }
-}
-
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_inject_phase(Grid<D> g, Ptr3 coords, uint8_t* index,
-                                                         int nargs, ArgPtrs args, int min_q,
-                                                         uint64_t seed, uint64_t counter, Off3 off,
-                                                         jp_stats* st) {
-    int64_t ncol[D];
-    int64_t total = 1;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        ncol[d] = (g.ni[d] + 1) / 2;
-        total *= ncol[d];
+    if (lane == 0) {
+        stat_add(st, ST_INJECTED, n_inj);
+        stat_add(st, ST_ORPHAN, n_orph);
     }
-    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    int64_t I[D];
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        I[d] = 2 * (t % ncol[d]) + off.o[d];
-        t /= ncol[d];
-        ok = ok && I[d] < g.ni[d];
-    }
-    if (!ok) return;
-    int64_t cnt[2] = {0, 0};
-    inject_cell<D>(g, coords, index, nargs, args, min_q, seed, counter, I, cnt);
-    stat_add(st, ST_INJECTED, cnt[0]);
-    stat_add(st, ST_ORPHAN, cnt[1]);
 }
 
 // ============================================================ host side
-template <int D>
-Vel<D> to_vel(const jp_velgrid* vg) {
-    Vel<D> V;
-    for (int c = 0; c < D; ++c)
-        for (int d = 0; d < D; ++d) {
-            const jp_axis& a = vg->ax[c][d];
-            V.ax[c][d] = Axis{a.x, a.dx, a.inv_dx, a.n};
-        }
-    return V;
-}
-
 int check_ptrs(jp_ctx* ctx, int n, const void* const* p, const char* what) {
     if (!p) return fail(ctx, JP_ERR_ARG, "null %s array", what);
     for (int i = 0; i < n; ++i)
@@ -612,61 +455,6 @@ int run_init(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, uint8_t* 
     return finish(ctx);
 }
 
-template <int D, int M>
-int launch_advect(jp_ctx* ctx, const Grid<D>& G, const Vel<D>& vg, const VPtrs<D>& V, Ptr3 cp,
-                  const uint8_t* index, const RKParams& P) {
-    dim3 grid((unsigned)blocks_for(G.C), (unsigned)G.K);
-    k_advect<D, M><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P);
-    JP_LAUNCHED(ctx);
-    return JP_OK;
-}
-
-template <int D>
-int run_advect(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, const uint8_t* index,
-               const double* const* Vp, const jp_velgrid* vgp, int method, const RKParams& P) {
-    Grid<D> G = to_grid<D>(g);
-    Vel<D> vg = to_vel<D>(vgp);
-    VPtrs<D> V;
-    for (int c = 0; c < D; ++c) V.v[c] = Vp[c];
-    Ptr3 cp = ptr3(coords, D);
-    int rc;
-    if (method == JP_EULER)
-        rc = launch_advect<D, JP_EULER>(ctx, G, vg, V, cp, index, P);
-    else if (method == JP_RK2)
-        rc = launch_advect<D, JP_RK2>(ctx, G, vg, V, cp, index, P);
-    else
-        rc = launch_advect<D, JP_RK4>(ctx, G, vg, V, cp, index, P);
-    if (rc) return rc;
-    return finish(ctx);
-}
-
-template <int D>
-int run_move(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, uint8_t* index, int nargs,
-             double* const* args, jp_stats* stats) {
-    Grid<D> G = to_grid<D>(g);
-    int* flag = dflags(ctx);
-    int rc = zero_bytes(ctx, flag, sizeof(int));
-    if (rc) return rc;
-    if ((rc = stats_begin(ctx, stats))) return rc;
-    dim3 grid((unsigned)blocks_for(G.C), (unsigned)G.K);
-    CPtr3 cc = cptr3(coords, D);
-    k_move_far<D><<<grid, kBlock, 0, ctx->stream>>>(G, cc, index, flag);
-    JP_LAUNCHED(ctx);
-    int64_t total = 1;
-    for (int d = 0; d < D; ++d) total *= (G.ni[d] + 2) / 3;
-    const int nphase = D == 2 ? 9 : 27;
-    Ptr3 cp = ptr3(coords, D);
-    ArgPtrs ap = argptrs(args, nargs);
-    jp_stats* st = stats ? dstats(ctx) : nullptr;
-    for (int ph = 0; ph < nphase; ++ph) {
-        k_move_phase<D><<<blocks_for(total), kBlock, 0, ctx->stream>>>(
-            G, cp, index, nargs, ap, phase_offsets(D, ph, 3), flag, st);
-        JP_LAUNCHED(ctx);
-    }
-    if ((rc = stats_end(ctx, stats))) return rc;
-    return finish(ctx);
-}
-
 template <int D>
 int run_p2g(jp_ctx* ctx, const jp_cellgrid* g, int nf, double* const* F, const double* const* Fp,
             const double* const* coords, const uint8_t* index, int mode) {
@@ -689,53 +477,45 @@ template <int D>
 int run_g2p(jp_ctx* ctx, const jp_cellgrid* g, double* Fp, const double* F,
             const double* const* coords, const uint8_t* index) {
     Grid<D> G = to_grid<D>(g);
-    dim3 grid((unsigned)blocks_for(G.C), (unsigned)G.K);
+    dim3 grid((unsigned)row_blocks<D>(G), (unsigned)G.K);
     k_g2p<D><<<grid, kBlock, 0, ctx->stream>>>(G, Fp, F, cptr3(coords, D), index);
     JP_LAUNCHED(ctx);
     return finish(ctx);
 }
 
 template <int D>
-int run_inject(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, uint8_t* index, int nargs,
-               double* const* args, int min_xcell, uint64_t seed, uint64_t counter,
+int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_t* index,
+               int nargs, double* const* args, int min_xcell, uint64_t seed, uint64_t counter,
                jp_stats* stats) {
-    Grid<D> G = to_grid<D>(g);
+    Grid<D> G = to_grid<D>(gc);
     int rc;
     if ((rc = stats_begin(ctx, stats))) return rc;
     constexpr int NQ = D == 2 ? 4 : 8;
-    int min_q = (min_xcell + NQ - 1) / NQ;
-    int64_t total = 1;
-    for (int d = 0; d < D; ++d) total *= (G.ni[d] + 1) / 2;
+    const int min_q = (min_xcell + NQ - 1) / NQ;
+    // cells per colour: at most prod(ceil(ni/2))
+    int64_t cap = 1;
+    for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
+    const size_t counts_bytes = 256;
+    if ((rc = ensure_scratch(ctx, counts_bytes + (size_t)NQ * cap * 4))) return rc;
+    int32_t* counts = static_cast<int32_t*>(ctx->scratch);
+    int32_t* lists = reinterpret_cast<int32_t*>(static_cast<char*>(ctx->scratch) + counts_bytes);
+    if ((rc = zero_bytes(ctx, counts, counts_bytes))) return rc;
+    k_inject_scan<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D), index,
+                                                                  min_q, lists, counts, cap);
+    JP_LAUNCHED(ctx);
     Ptr3 cp = ptr3(coords, D);
     ArgPtrs ap = argptrs(args, nargs);
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     for (int ph = 0; ph < NQ; ++ph) {
-        k_inject_phase<D><<<blocks_for(total), kBlock, 0, ctx->stream>>>(
-            G, cp, index, nargs, ap, min_q, seed, counter, phase_offsets(D, ph, 2), st);
+        // colour index of the phase: offsets (o_1 outer ... o_D inner)
+        // one wavefront per listed cell (kBlock/64 cells per block)
+        k_inject_wave<D><<<blocks_for(cap, kBlock / 64), kBlock, 0, ctx->stream>>>(
+            G, cp, index, nargs, ap, min_q, seed, counter, lists + (int64_t)ph * cap,
+            counts + ph, st);
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);
-}
-
-int check_velgrid(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
-                  const double* const* V) {
-    if (!vg) return fail(ctx, JP_ERR_ARG, "null velocity grid%s");
-    if (!V) return fail(ctx, JP_ERR_ARG, "null velocity array%s");
-    for (int c = 0; c < g->ndim; ++c) {
-        if (!V[c]) return fail(ctx, JP_ERR_ARG, "null velocity component%s %lld", "", c);
-        for (int d = 0; d < g->ndim; ++d) {
-            const jp_axis& a = vg->ax[c][d];
-            if (!a.x || !a.dx || !a.inv_dx)
-                return fail(ctx, JP_ERR_ARG, "null velocity axis pointer%s (component %lld)", "", c);
-            // the bisection is seeded with the particle's cell index, which
-            // must address a bracket [x[s], x[s+1]] of the velocity axis
-            if (a.n < g->ni[d] + 1)
-                return fail(ctx, JP_ERR_ARG,
-                            "velocity axis shorter than the cell grid%s (component %lld)", "", c);
-        }
-    }
-    return JP_OK;
 }
 
 }  // namespace
@@ -753,41 +533,6 @@ int jp_init_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, 
         return fail(ctx, JP_ERR_ARG, "np_quadrant*2^D exceeds max_xcell%s");
     return g->ndim == 2 ? run_init<2>(ctx, g, coords, index, np_quadrant, seed)
                         : run_init<3>(ctx, g, coords, index, np_quadrant, seed);
-}
-
-int jp_advection(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, const uint8_t* index,
-                 const double* const* V, const jp_velgrid* vg, int method, double alpha,
-                 double dt) {
-    int rc = check_grid(ctx, g);
-    if (rc) return rc;
-    if ((rc = check_ptrs(ctx, g->ndim, (const void* const*)coords, "coords"))) return rc;
-    if (!index) return fail(ctx, JP_ERR_ARG, "null index%s");
-    if ((rc = check_velgrid(ctx, g, vg, V))) return rc;
-    if (method < JP_EULER || method > JP_RK4)
-        return fail(ctx, JP_ERR_ARG, "unknown integrator%s %lld", "", method);
-    if (method == JP_RK2 && !(0.0 < alpha && alpha < 1.0))
-        return fail(ctx, JP_ERR_ARG, "Only 0 < α < 1 is supported%s");
-    RKParams P;
-    P.dt = dt;
-    P.adt = (1.0 * alpha) * dt;
-    P.half = alpha == 0.5;
-    double ia = 1.0 / alpha;
-    P.w0 = std::fma(-0.5, ia, 1.0);
-    P.w1 = 0.5 * ia;
-    return g->ndim == 2 ? run_advect<2>(ctx, g, coords, index, V, vg, method, P)
-                        : run_advect<3>(ctx, g, coords, index, V, vg, method, P);
-}
-
-int jp_move_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, uint8_t* index,
-                      int nargs, double* const* args, jp_stats* stats) {
-    int rc = check_grid(ctx, g);
-    if (rc) return rc;
-    if ((rc = check_ptrs(ctx, g->ndim, (const void* const*)coords, "coords"))) return rc;
-    if (!index) return fail(ctx, JP_ERR_ARG, "null index%s");
-    if (nargs < 0 || nargs > JP_MAX_ARGS) return fail(ctx, JP_ERR_ARG, "nargs out of range%s");
-    if (nargs && (rc = check_ptrs(ctx, nargs, (const void* const*)args, "args"))) return rc;
-    return g->ndim == 2 ? run_move<2>(ctx, g, coords, index, nargs, args, stats)
-                        : run_move<3>(ctx, g, coords, index, nargs, args, stats);
 }
 
 int jp_particle2grid(jp_ctx* ctx, const jp_cellgrid* g, double* F, const double* Fp,

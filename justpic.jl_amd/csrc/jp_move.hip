@@ -1,0 +1,532 @@
+// jp_move.hip -- move_particles! (src/Particles/move_safe.jl:23-125) on gfx950.
+//
+// The reference visits cells in 3^D colour phases; inside a cell it walks the
+// slots in order, and a particle that left its cell goes to the first free
+// slot >= starting_point of its destination (starting_point shared by all
+// destinations of one source cell).  The outcome depends only on which slots
+// are occupied, so it is computed on 64-bit occupancy words and the particle
+// data are moved afterwards in one parallel pass:
+//
+//   1. classify (one thread per cell, all cells, coalesced over the cell
+//      planes): in-cell test, domain test, destination (bisection seeded at
+//      the cell) as a neighbour code, occupancy/mover words, and a copy of
+//      every mover's coordinates and fields into a staging area indexed by
+//      its original slot;
+//   2. 3^D colour phases (one thread per source cell of the colour): the
+//      reference's slot loop replayed on the occupancy words of the 3^D
+//      neighbourhood (kept in LDS), recording for every slot that received a
+//      particle where that particle came from;
+//   3. scatter (one thread per cell): arrivals copy their staged data into
+//      their final slot, vacated slots get NaN and mask false.
+//
+// A particle that jumps more than one cell breaks the colour independence;
+// classify raises a flag and a single-thread kernel then runs the reference
+// loop for every phase in order (the other kernels return immediately).
+#include "jp_ctx.hpp"
+
+using namespace jp;
+
+namespace {
+
+constexpr uint8_t kStay = 0xFF;
+constexpr uint8_t kDel = 0xFE;
+constexpr uint8_t kGap = 0x80;  // arrival fails the strict in-cell test of its destination
+
+__device__ inline double dnan() { return __builtin_nan(""); }
+
+struct MoveScratch {
+    uint64_t* occ0;  // occupancy before the call
+    uint64_t* occ;   // running occupancy
+    uint64_t* mov;   // slots holding a particle that moves when its cell's phase runs
+    uint64_t* arr;   // slots holding a particle that arrived during this call
+    uint64_t* tch;   // slots that received a particle at some point
+    uint8_t* code;   // per slot (plane-major c + C*k): neighbour code / kDel, movers only
+    uint32_t* org;   // per slot (plane-major): original slot index of an arrival
+    double* stg;     // (D + nargs) planes of C*K: staged mover data by original slot
+};
+
+template <int D>
+__device__ inline bool in_cell_at(const Grid<D>& g, const int64_t* I, const double* p) {
+    bool b = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double x = g.xv[d].x[I[d]], h = g.xv[d].dx[I[d]];
+        b = b & (x < p[d] && p[d] < x + h);  // isincell, src/Particles/utils.jl:15
+    }
+    return b;
+}
+
+template <int D>
+__device__ inline bool in_domain(const Grid<D>& g, const double* p) {
+#pragma unroll
+    for (int d = 0; d < D; ++d)  // indomain, src/Particles/move_safe.jl:199-206
+        if (!(g.xv[d].x[0] < p[d] && p[d] < g.xv[d].x[g.xv[d].n - 1])) return false;
+    return true;
+}
+
+template <int D>
+constexpr int kNB = D == 2 ? 9 : 27;
+template <int D>
+constexpr int kCenter = (kNB<D> - 1) / 2;
+
+// colour phase of a cell: offsets (o_1 outer ... o_D inner), 0-based
+template <int D>
+__device__ inline int phase_of(const int64_t* I) {
+    int ph = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) ph = ph * 3 + (int)(I[d] % 3);
+    return ph;
+}
+
+// ------------------------------------------------------------ 1. classify
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
+                                                        const uint8_t* __restrict__ index,
+                                                        int nargs, ArgPtrs args, MoveScratch ms,
+                                                        int* flag) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int64_t C = g.C;
+    int64_t I[D];
+    unlin<D>(g.ni, c, I);
+    const uint64_t occ = active_slots(index, c, C, g.K);
+    uint64_t mov = 0;
+    // cell geometry (isincell uses the scalar/vector vertex spacing)
+    double lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        lo[d] = g.xv[d].x[I[d]];
+        hi[d] = lo[d] + g.xv[d].dx[I[d]];
+    }
+    // chunks of slot planes walked in lockstep by the wavefront (coalesced
+    // loads and stores across neighbouring cells)
+    constexpr int B = 8;
+    for (int k0 = 0; k0 < g.K; k0 += B) {
+    double pb[B][D];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const bool on = (occ >> (k0 + j)) & 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + C * (k0 + j)] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const int k = k0 + j;
+        if (!((occ >> k) & 1)) continue;
+        double p[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[d] = pb[j][d];
+        bool inc = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) inc = inc & (lo[d] < p[d] && p[d] < hi[d]);
+        if (inc) continue;
+        mov |= 1ull << k;
+        const int64_t e = c + C * k;
+        uint8_t cd;
+        if (!in_domain<D>(g, p)) {
+            cd = kDel;
+        } else {
+            int n = 0, mult = 1;
+            bool far = false;
+            int64_t J[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                J[d] = find_cell(p[d], g.xv[d], I[d]);
+                int64_t dl = J[d] - I[d];
+                far = far || dl > 1 || dl < -1;
+                n += (int)(dl + 1) * mult;
+                mult *= 3;
+            }
+            if (far) {
+                atomicOr(flag, 1);
+                cd = kDel;  // unused: the serial kernel takes over
+            } else {
+                cd = (uint8_t)n;
+                if (n != kCenter<D> && !in_cell_at<D>(g, J, p)) cd |= kGap;
+#pragma unroll
+                for (int d = 0; d < D; ++d) ms.stg[d * (C * g.K) + e] = p[d];
+                for (int a = 0; a < nargs; ++a) ms.stg[(D + a) * (C * g.K) + e] = args.p[a][e];
+            }
+        }
+        ms.code[e] = cd;  // plane-major, read back only for mover slots
+    }
+    }
+    ms.occ0[c] = occ;
+    ms.occ[c] = occ;
+    ms.mov[c] = mov;
+    ms.arr[c] = 0;
+    ms.tch[c] = 0;
+}
+
+// ------------------------------------------------------------ 2. colour phases
+template <int D>
+__device__ inline int64_t nb_cell(const Grid<D>& g, const int64_t* I, int n, bool& ok) {
+    int64_t J[D];
+    ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        J[d] = I[d] + (n % 3) - 1;
+        n /= 3;
+        ok = ok && J[d] >= 0 && J[d] < g.ni[d];
+    }
+    return ok ? lin<D>(g.ni, J) : -1;
+}
+
+template <int D, int BS>
+__global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3 off, int phase,
+                                                 const int* __restrict__ flag, jp_stats* st) {
+    __shared__ uint64_t s_occ[kNB<D>][BS];
+    if (*flag) return;
+    int64_t ncol[D];
+    int64_t total = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ncol[d] = (g.ni[d] + 2) / 3;
+        total *= ncol[d];
+    }
+    int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
+    if (t >= total) return;
+    int64_t I[D];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I[d] = 3 * (t % ncol[d]) + off.o[d];
+        t /= ncol[d];
+        ok = ok && I[d] < g.ni[d];
+    }
+    if (!ok) return;
+    const int64_t S = lin<D>(g.ni, I);
+    const int tid = threadIdx.x;
+    const int K = g.K;
+    const int64_t C = g.C;
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    // everything this cell needs, in one round of independent loads
+    uint64_t pend = ms.mov[S];
+    uint64_t arrS = ms.arr[S];
+    uint64_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // movers' code bytes, 8 per word
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        if (j < K && ((pend >> j) & 1)) cw[j >> 3] |= (uint64_t)ms.code[S + C * j] << (8 * (j & 7));
+    int64_t nbc[kNB<D>];
+#pragma unroll
+    for (int n = 0; n < kNB<D>; ++n) {
+        bool v;
+        nbc[n] = nb_cell<D>(g, I, n, v);
+        s_occ[n][tid] = v ? ms.occ[nbc[n]] : ~0ull;  // outside the grid: never free
+    }
+    if (!pend) return;  // nothing leaves or re-slots in this cell
+    uint64_t selfm = 0;  // slots that received a re-slotted particle of this cell
+    uint64_t selfins = 0;
+    int sp = 0;
+    int64_t n_moved = 0, n_del = 0, n_drop = 0;
+    while (pend) {
+        const int ip = __builtin_ctzll(pend);
+        pend &= pend - 1;
+        const uint64_t bit = 1ull << ip;
+        uint64_t wsel = cw[0];
+#pragma unroll
+        for (int j = 1; j < 8; ++j) wsel = ((ip >> 3) == j) ? cw[j] : wsel;
+        const uint8_t cd =
+            (selfm & bit) ? (uint8_t)kCenter<D> : (uint8_t)(wsel >> (8 * (ip & 7)));
+        const uint32_t origin = (arrS & bit) ? ms.org[S + C * ip] : (uint32_t)(S + C * ip);
+        s_occ[kCenter<D>][tid] &= ~bit;  // empty_particle! on the source slot
+        arrS &= ~bit;
+        selfm &= ~bit;
+        if (cd == kDel) {
+            ++n_del;
+            continue;
+        }
+        const int n = cd & 0x7F;
+        const uint64_t od = s_occ[n][tid];
+        const uint64_t fr = ~od & full & (~0ull << sp);
+        if (!fr) {  // find_free_memory found nothing: dropped
+            ++n_drop;
+            continue;
+        }
+        const int f = __builtin_ctzll(fr);
+        sp = f;  // starting_point = free_idx
+        s_occ[n][tid] = od | (1ull << f);
+        ++n_moved;
+        int64_t Dc = S;
+        {
+            int r = n;
+            int64_t stride = 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                Dc += (int64_t)(r % 3 - 1) * stride;
+                r /= 3;
+                stride *= g.ni[d];
+            }
+        }
+        ms.org[Dc + C * f] = origin;
+        if (n == kCenter<D>) {
+            arrS |= 1ull << f;
+            selfins |= 1ull << f;
+            if (f > ip) {  // the slot loop will reach this particle again
+                pend |= 1ull << f;
+                selfm |= 1ull << f;
+            }
+        } else if (cd & kGap) {
+            // the arrival fails the strict test of its new cell: when that cell's
+            // phase comes later, it is re-slotted there (bisection returns that cell)
+            int64_t J[D];
+            unlin<D>(g.ni, Dc, J);
+            if (phase_of<D>(J) > phase) {
+                ms.mov[Dc] |= 1ull << f;
+                ms.code[Dc + C * f] = (uint8_t)kCenter<D>;
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < kNB<D>; ++n) {
+        if (n == kCenter<D> || nbc[n] < 0) continue;
+        const uint64_t now = s_occ[n][tid];
+        const uint64_t was = ms.occ[nbc[n]];
+        if (now != was) {
+            const uint64_t ins = now & ~was;
+            ms.occ[nbc[n]] = now;
+            ms.arr[nbc[n]] |= ins;
+            ms.tch[nbc[n]] |= ins;
+        }
+    }
+    ms.occ[S] = s_occ[kCenter<D>][tid];
+    ms.arr[S] = arrS;
+    ms.tch[S] |= selfins;
+    ms.mov[S] = 0;
+    stat_add(st, ST_MOVED, n_moved);
+    stat_add(st, ST_DELETED, n_del);
+    stat_add(st, ST_DROPPED, n_drop);
+}
+
+// ------------------------------------------------------------ 3. scatter
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, Ptr3 coords,
+                                                       uint8_t* __restrict__ index, int nargs,
+                                                       ArgPtrs args, MoveScratch ms,
+                                                       const int* __restrict__ flag) {
+    // one thread per slot; a wavefront covers 64 neighbouring cells of one
+    // slot plane (blockIdx.y), so the per-cell words, origins and the final
+    // stores are coalesced; only the staged-data reads are scattered.
+    if (*flag) return;
+    int64_t c, I[D];
+    if (!row_cell<D>(g, c, I)) return;
+    const int k = blockIdx.y;
+    const uint64_t bit = 1ull << k;
+    const uint64_t arr = ms.arr[c];
+    const bool arrived = arr & bit;
+    if (!arrived) {
+        const uint64_t occF = ms.occ[c];
+        if (occF & bit) return;  // untouched particle
+        if (!((ms.occ0[c] | ms.tch[c]) & bit)) return;  // was and stays empty
+    }
+    const int64_t C = g.C, S = C * g.K;
+    const int64_t e = c + C * k;
+    if (arrived) {
+        const int64_t o = ms.org[e];
+#pragma unroll
+        for (int d = 0; d < D; ++d) coords.p[d][e] = ms.stg[d * S + o];
+        for (int a = 0; a < nargs; ++a) args.p[a][e] = ms.stg[(D + a) * S + o];
+        index[e] = 1;
+    } else {  // vacated: empty_particle! (src/Particles/move_safe.jl:243-251)
+#pragma unroll
+        for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
+        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
+        index[e] = 0;
+    }
+}
+
+// ------------------------------------------------------------ serial path
+// move_kernel! (src/Particles/move_safe.jl:72-125) for one cell, directly on
+// the data (used when some particle jumps more than one cell).
+template <int D>
+__device__ void move_cell_serial(const Grid<D>& g, const Ptr3& coords, uint8_t* index, int nargs,
+                                 const ArgPtrs& args, const int64_t* I, int64_t (&cnt)[4]) {
+    const int64_t C = g.C;
+    const int K = g.K;
+    const int64_t c = lin<D>(g.ni, I);
+    int sp = 0;
+    for (int ip = 0; ip < K; ++ip) {
+        const int64_t e = c + C * ip;
+        if (!index[e]) continue;
+        double p[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
+        if (in_cell_at<D>(g, I, p)) continue;
+        if (!in_domain<D>(g, p)) {
+            index[e] = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
+            for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
+            cnt[1]++;
+            continue;
+        }
+        int64_t nc[D];
+        bool far = false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            nc[d] = find_cell(p[d], g.xv[d], I[d]);
+            far = far || (nc[d] - I[d] > 1 || I[d] - nc[d] > 1);
+        }
+        if (far) cnt[3]++;
+        double cached[JP_MAX_ARGS];
+        for (int a = 0; a < nargs; ++a) cached[a] = args.p[a][e];
+        index[e] = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
+        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
+        const int64_t cn = lin<D>(g.ni, nc);
+        int fi = -1;
+        for (int i = sp; i < K; ++i)
+            if (!index[cn + C * i]) {
+                fi = i;
+                break;
+            }
+        if (fi < 0) {
+            cnt[2]++;
+            continue;
+        }
+        sp = fi;
+        const int64_t f = cn + C * fi;
+        index[f] = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) coords.p[d][f] = p[d];
+        for (int a = 0; a < nargs; ++a) args.p[a][f] = cached[a];
+        cnt[0]++;
+    }
+}
+
+template <int D>
+__global__ void k_mv_serial(Grid<D> g, Ptr3 coords, uint8_t* index, int nargs, ArgPtrs args,
+                            const int* __restrict__ flag, jp_stats* st) {
+    if (!*flag || blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t ncol[D];
+    int64_t total = 1;
+    for (int d = 0; d < D; ++d) {
+        ncol[d] = (g.ni[d] + 2) / 3;
+        total *= ncol[d];
+    }
+    int64_t cnt[4] = {0, 0, 0, 0};
+    const int nph = kNB<D>;
+    for (int ph = 0; ph < nph; ++ph) {
+        int o[D];
+        int r = ph;
+        for (int d = D - 1; d >= 0; --d) {
+            o[d] = r % 3;
+            r /= 3;
+        }
+        for (int64_t t = 0; t < total; ++t) {
+            int64_t q = t, I[D];
+            bool ok = true;
+            for (int d = 0; d < D; ++d) {
+                I[d] = 3 * (q % ncol[d]) + o[d];
+                q /= ncol[d];
+                ok = ok && I[d] < g.ni[d];
+            }
+            if (ok) move_cell_serial<D>(g, coords, index, nargs, args, I, cnt);
+        }
+    }
+    stat_add(st, ST_MOVED, cnt[0]);
+    stat_add(st, ST_DELETED, cnt[1]);
+    stat_add(st, ST_DROPPED, cnt[2]);
+    stat_add(st, ST_FAR, cnt[3]);
+}
+
+Ptr3 ptr3(double* const* c, int D) {
+    Ptr3 p{{nullptr, nullptr, nullptr}};
+    for (int d = 0; d < D; ++d) p.p[d] = c[d];
+    return p;
+}
+CPtr3 cptr3(double* const* c, int D) {
+    CPtr3 p{{nullptr, nullptr, nullptr}};
+    for (int d = 0; d < D; ++d) p.p[d] = c[d];
+    return p;
+}
+ArgPtrs argptrs(double* const* a, int n) {
+    ArgPtrs p;
+    for (int i = 0; i < JP_MAX_ARGS; ++i) p.p[i] = i < n ? a[i] : nullptr;
+    return p;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <int D>
+int run_move(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_t* index, int nargs,
+             double* const* args, jp_stats* stats) {
+    Grid<D> g = to_grid<D>(gc);
+    const int64_t C = g.C, S = C * g.K;
+    if (S >= ((int64_t)1 << 32))
+        return fail(ctx, JP_ERR_ARG, "more than 2^32 slots%s");
+    MoveScratch ms;
+    size_t off = 0;
+    const size_t o_occ0 = off; off = align_up(off + C * 8, 256);
+    const size_t o_occ = off;  off = align_up(off + C * 8, 256);
+    const size_t o_mov = off;  off = align_up(off + C * 8, 256);
+    const size_t o_arr = off;  off = align_up(off + C * 8, 256);
+    const size_t o_tch = off;  off = align_up(off + C * 8, 256);
+    const size_t o_code = off; off = align_up(off + (size_t)S, 256);
+    const size_t o_org = off;  off = align_up(off + (size_t)S * 4, 256);
+    const size_t o_stg = off;  off = align_up(off + (size_t)S * 8 * (D + nargs), 256);
+    int rc = ensure_scratch(ctx, off);
+    if (rc) return rc;
+    char* base = static_cast<char*>(ctx->scratch);
+    ms.occ0 = (uint64_t*)(base + o_occ0);
+    ms.occ = (uint64_t*)(base + o_occ);
+    ms.mov = (uint64_t*)(base + o_mov);
+    ms.arr = (uint64_t*)(base + o_arr);
+    ms.tch = (uint64_t*)(base + o_tch);
+    ms.code = (uint8_t*)(base + o_code);
+    ms.org = (uint32_t*)(base + o_org);
+    ms.stg = (double*)(base + o_stg);
+
+    int* flag = dflags(ctx);
+    if ((rc = zero_bytes(ctx, flag, sizeof(int)))) return rc;
+    if ((rc = stats_begin(ctx, stats))) return rc;
+    jp_stats* st = stats ? dstats(ctx) : nullptr;
+    ArgPtrs ap = argptrs(args, nargs);
+    k_mv_classify<D><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cptr3(coords, D), index, nargs,
+                                                               ap, ms, flag);
+    JP_LAUNCHED(ctx);
+    int64_t total = 1;
+    for (int d = 0; d < D; ++d) total *= (g.ni[d] + 2) / 3;
+    constexpr int BS = D == 2 ? 256 : 128;
+    const int nphase = kNB<D>;
+    for (int ph = 0; ph < nphase; ++ph) {
+        Off3 o{{0, 0, 0}};
+        int r = ph;
+        for (int d = D - 1; d >= 0; --d) {
+            o.o[d] = r % 3;
+            r /= 3;
+        }
+        k_mv_phase<D, BS><<<(unsigned)((total + BS - 1) / BS), BS, 0, ctx->stream>>>(g, ms, o, ph,
+                                                                                    flag, st);
+        JP_LAUNCHED(ctx);
+    }
+    k_mv_serial<D><<<1, 64, 0, ctx->stream>>>(g, ptr3(coords, D), index, nargs, ap, flag, st);
+    JP_LAUNCHED(ctx);
+    k_mv_scatter<D><<<dim3((unsigned)row_blocks<D>(g), (unsigned)g.K), kBlock, 0, ctx->stream>>>(
+        g, ptr3(coords, D), index, nargs, ap, ms, flag);
+    JP_LAUNCHED(ctx);
+    if ((rc = stats_end(ctx, stats))) return rc;
+    return finish(ctx);
+}
+
+int check_ptrs(jp_ctx* ctx, int n, const void* const* p, const char* what) {
+    if (!p) return fail(ctx, JP_ERR_ARG, "null %s array", what);
+    for (int i = 0; i < n; ++i)
+        if (!p[i]) return fail(ctx, JP_ERR_ARG, "null %s pointer %lld", what, i);
+    return JP_OK;
+}
+
+}  // namespace
+
+extern "C" int jp_move_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                                 uint8_t* index, int nargs, double* const* args, jp_stats* stats) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if ((rc = check_ptrs(ctx, g->ndim, (const void* const*)coords, "coords"))) return rc;
+    if (!index) return fail(ctx, JP_ERR_ARG, "null index%s");
+    if (nargs < 0 || nargs > JP_MAX_ARGS) return fail(ctx, JP_ERR_ARG, "nargs out of range%s");
+    if (nargs && (rc = check_ptrs(ctx, nargs, (const void* const*)args, "args"))) return rc;
+    return g->ndim == 2 ? run_move<2>(ctx, g, coords, index, nargs, args, stats)
+                        : run_move<3>(ctx, g, coords, index, nargs, args, stats);
+}
