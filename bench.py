@@ -8,6 +8,11 @@ and per step: advection! (RK2), move_particles!, inject_particles!,
 particle2grid! to the 513^2 nodal T.  A step is one pass of that sequence over
 all particles; inputs are resident in HBM before timing starts.
 
+The reported value is the FP32-split form (``--form split``); the FP64
+drop-in runs the same workload beside it (``compare``), and a separate
+advection + move run from one initial state measures the split form's
+position error against FP64 and its cell-assignment disagreements.
+
 Prints one JSON line (metric particle-steps/s).  Progress goes to stderr.
 """
 import argparse
@@ -31,10 +36,11 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(S, P, D, M, F, Nn, Vb, injected=0):
+def algorithmic_bytes(S, P, D, M, F, Nn, Vb, injected=0, w=8):
     """Byte model per step: each kernel that touches an array counts it once.
-    m = 1 B mask/slot, w = 8 B coordinate, wf = 8 B field."""
-    m, w, wf = 1, 8, 8
+    m = 1 B mask/slot, w = coordinate bytes (8 FP64, 4 FP32-split offset),
+    wf = 8 B field; Vb = FP64 velocity bytes (the input in both forms)."""
+    m, wf = 1, 8
     adv = S * m + 2 * P * D * w + Vb
     move = S * m + P * D * w + M * (2 * D * w + 3 * F * wf + 2 * m)
     inj = S * m + P * D * w + injected * (D * w + F * wf + m)
@@ -65,72 +71,59 @@ def cpu_baseline(cfg, n_steps, seed):
     return psteps / el, el
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=512, help="cells per dimension")
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=6,
-                    help="oracle steps timed for the CPU baseline (0 = skip)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "see the sharding design notes", file=sys.stderr)
-        sys.exit(2)
+CALLS = ("advection", "move", "inject", "p2g")
 
-    import justpic_hip as jp
-    from justpic_hip.configs import gaussian_T, stream2d
 
-    log("building workload", args.n)
-    cfg = stream2d(args.n, amp=250.0, readme=False)
+def run_form(jp, ctx, cfg, form, args):
+    """Warm up, then time args.steps full steps of one position form.  Every
+    call is bracketed by hipEvents on the context stream."""
+    from justpic_hip.configs import gaussian_T
     xv, xi, V, dt = cfg["xv"], cfg["xi_vel"], cfg["V"], cfg["dt"]
-    ctx = jp.Context(0, synchronous=False)
-    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed)
+    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed, form=form)
     Vd = tuple(jp.DeviceArray.from_host(ctx, v) for v in V)
     T = jp.DeviceArray.from_host(ctx, gaussian_T(xv))
     pT = jp.CellArray(ctx, p.ni, p.max_xcell, fill=float("nan"))
     jp.grid2particle_(pT, T, p)
     rk2 = jp.RungeKutta2()
     ctx.sync()
+    calls = (lambda: jp.advection_(p, rk2, Vd, dt),
+             lambda: jp.move_particles_(p, (pT,)),
+             lambda: jp.inject_particles_(p, (pT,)),
+             lambda: jp.particle2grid_(T, pT, p))
 
-    ev = [ctx.event() for _ in range(2)]
+    def step(evs=None):
+        for i, f in enumerate(calls):
+            if evs is not None:
+                ctx.record(evs[2 * i])
+            f()
+            if evs is not None:
+                ctx.record(evs[2 * i + 1])
 
-    def step(timed_adv=None):
-        if timed_adv is not None:
-            ctx.record(timed_adv[0])
-        jp.advection_(p, rk2, Vd, dt)
-        if timed_adv is not None:
-            ctx.record(timed_adv[1])
-        jp.move_particles_(p, (pT,))
-        jp.inject_particles_(p, (pT,))
-        jp.particle2grid_(T, pT, p)
-
-    log("warmup", args.warmup)
+    log(form, "warmup", args.warmup)
     for _ in range(args.warmup):
         step()
     ctx.sync()
     P0 = p.active()
-    # timed region: events bracket every advection launch (same stream)
-    adv_ev = [(ctx.event(), ctx.event()) for _ in range(args.steps)]
-    log("timing", args.steps, "steps")
+    evs = [[ctx.event() for _ in range(2 * len(calls))] for _ in range(args.steps)]
+    e0, e1 = ctx.event(), ctx.event()
+    log(form, "timing", args.steps, "steps")
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.record(ev[0])
+    ctx.record(e0)
     for s in range(args.steps):
-        step(adv_ev[s])
-    ctx.record(ev[1])
+        step(evs[s])
+    ctx.record(e1)
     ctx.sync()
     wall = time.perf_counter() - t0
-    gpu_ms = ctx.elapsed_ms(ev[0], ev[1])
+    gpu_ms = ctx.elapsed_ms(e0, e1)
     P1 = p.active()
-    adv_ms = float(np.mean([ctx.elapsed_ms(a, b) for a, b in adv_ev]))
+    call_ms = {name: float(np.mean([ctx.elapsed_ms(ev[2 * i], ev[2 * i + 1]) for ev in evs]))
+               for i, name in enumerate(CALLS)}
     ms_per_step = gpu_ms / args.steps
     P = 0.5 * (P0 + P1)
     value = P * args.steps / (gpu_ms * 1e-3)
-    log(f"gpu {gpu_ms:.2f} ms (host wall {wall * 1e3:.2f} ms), {ms_per_step:.4f} ms/step, "
-        f"P {P0}->{P1}, advection {adv_ms * 1e3:.1f} us")
+    log(f"{form}: gpu {gpu_ms:.2f} ms (host wall {wall * 1e3:.2f} ms), {ms_per_step:.4f} ms/step,"
+        f" P {P0}->{P1}, " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in call_ms.items()))
 
     # one calibration step with outcome counters (not timed) for the byte model
     jp.advection_(p, rk2, Vd, dt)
@@ -141,12 +134,95 @@ def main():
     Nn = int(np.prod([n + 1 for n in p.ni]))
     Vb = sum(v.nbytes for v in Vd)
     B = algorithmic_bytes(S, P, 2, st_m["moved"] + st_m["deleted"], 1, Nn, Vb,
-                          st_i["injected"])
-    step_gbs = B["step"] / (ms_per_step * 1e-3) / 1e9
-    adv_gbs = B["advection"] / (adv_ms * 1e-3) / 1e9
+                          st_i["injected"], w=4 if form == "split" else 8)
+    calls_out = {k: {"us": call_ms[k] * 1e3, "bytes": B[k],
+                     "GBs": B[k] / (call_ms[k] * 1e-3) / 1e9} for k in CALLS}
+    return {"value": value, "ms_per_step": ms_per_step, "P": P, "bytes": B,
+            "calls": calls_out, "moved": st_m["moved"], "injected": st_i["injected"],
+            "hbm_frac_step": B["step"] / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
+
+def split_error(jp, ctx, cfg, args):
+    """Advection + move for args.check_steps steps from one initial state in
+    both forms, particles tracked by an id field: max position error of the
+    FP32-split form against the FP64 form (bit-identical to the FP64 oracle in
+    the parity tests), and the number of cell-assignment disagreements."""
+    xi, V, dt = cfg["xi_vel"], cfg["V"], cfg["dt"]
+    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed)
+    s = jp.to_split(p)
+    Vd = tuple(jp.DeviceArray.from_host(ctx, v) for v in V)
+    idx0 = p.index.data.flat()
+    ids0 = np.where(idx0 == 1, np.arange(idx0.size, dtype=np.float64), np.nan)
+    id_p = jp.CellArray(ctx, p.ni, p.max_xcell)
+    id_p.copy_from(ids0)
+    id_s = jp.CellArray(ctx, p.ni, p.max_xcell)
+    id_s.copy_from(ids0)
+    rk2 = jp.RungeKutta2()
+    for _ in range(args.check_steps):
+        jp.advection_(p, rk2, Vd, dt)
+        jp.move_particles_(p, (id_p,))
+        jp.advection_(s, rk2, Vd, dt)
+        jp.move_particles_(s, (id_s,))
+    ctx.sync()
+    C = p.C
+
+    def table(coords, index, ids):
+        e = np.nonzero(index == 1)[0]
+        iv = ids[e].astype(np.int64)
+        o = np.argsort(iv)
+        return iv[o], (e % C)[o], np.stack([c[e] for c in coords], 1)[o]
+
+    ca, ia = p.host_state()
+    cb, ib = jp.to_f64(s).host_state()
+    ka, cella, pa = table(ca, ia, id_p.data.flat())
+    kb, cellb, pb = table(cb, ib, id_s.data.flat())
+    common, ja, jb = np.intersect1d(ka, kb, assume_unique=True, return_indices=True)
+    err = float(np.max(np.abs(pa[ja] - pb[jb])))
+    dis = int(len(ka) + len(kb) - 2 * common.size + np.count_nonzero(cella[ja] != cellb[jb]))
+    return {"steps": args.check_steps, "particles": int(common.size), "max_abs_err": err,
+            "domain": 1.0, "cell_disagreements": dis}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="cells per dimension")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--form", choices=("split", "fp64"), default="split",
+                    help="position form whose throughput is the reported value")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the other form and the error check")
+    ap.add_argument("--check-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=6,
+                    help="oracle steps timed for the CPU baseline (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU sharding is a design note, not a runnable path", file=sys.stderr)
+        sys.exit(2)
+
+    import justpic_hip as jp
+    from justpic_hip.configs import stream2d
+
+    log("building workload", args.n)
+    cfg = stream2d(args.n, amp=250.0, readme=False)
+    ctx = jp.Context(0, synchronous=False)
+    main_r = run_form(jp, ctx, cfg, args.form, args)
+    other = None
+    err = None
+    if not args.no_compare:
+        other_form = "fp64" if args.form == "split" else "split"
+        other = run_form(jp, ctx, cfg, other_form, args)
+        log("split vs fp64 error check,", args.check_steps, "steps")
+        err = split_error(jp, ctx, cfg, args)
+        log("error", err)
+
+    # roofline: the advection kernel (one launch per call)
+    adv = main_r["calls"]["advection"]
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_advection_f64_{args.n}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_advection_{args.form}_{args.n}.json")
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
@@ -158,37 +234,49 @@ def main():
         log("cpu baseline: oracle,", args.cpu_steps, "steps at the same size")
         v_cpu, el = cpu_baseline(cfg, args.cpu_steps, args.seed)
         cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} full steps (advect RK2 + move + inject + p2g) of the "
-                         f"same {args.n}^2 workload by the sequential C oracle, "
+               "sample": f"{args.cpu_steps} full FP64 steps (advect RK2 + move + inject + p2g) "
+                         f"of the same {args.n}^2 workload by the sequential C oracle, "
                          f"{el:.1f} s"}
 
+    names = {"split": "FP32-split positions", "fp64": "FP64 drop-in"}
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_r["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.form == "split" else "f64",
         "data": "synthetic",
         "config": {"workload": f"2D temperature advection {args.n}^2 cells, nxcell 24, "
                                f"max_xcell 48, min_xcell 12, RK2 + move + inject + p2g, "
-                               f"FP64 drop-in",
-                   "cells": args.n * args.n, "particles": int(P), "form": "fp64"},
-        "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
-        "step_bytes": B["step"],
-        "roofline": {"bound": "hbm", "kernel": "advection (RK2)", "achieved": adv_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": adv_gbs / HBM_PEAK_GBS,
-                     "traffic": traffic, "bytes_per_launch": B["advection"],
-                     "avg_launch_us": adv_ms * 1e3},
+                               f"{names[args.form]}",
+                   "cells": args.n * args.n, "particles": int(main_r["P"]),
+                   "form": args.form},
+        "hbm_frac_step": main_r["hbm_frac_step"],
+        "step_bytes": main_r["bytes"]["step"],
+        "calls": main_r["calls"],
+        "roofline": {"bound": "hbm", "kernel": f"advection RK2 ({args.form})",
+                     "achieved": adv["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": adv["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": adv["bytes"], "avg_launch_us": adv["us"]},
         "cpu_baseline": cpu,
-        "movers_per_step": st_m["moved"],
-        "injected_per_step": st_i["injected"],
+        "movers_per_step": main_r["moved"],
+        "injected_per_step": main_r["injected"],
     }
+    if other is not None:
+        other_form = "fp64" if args.form == "split" else "split"
+        out["compare"] = {"form": other_form, "value": other["value"],
+                          "ms_per_step": other["ms_per_step"], "calls": other["calls"],
+                          "hbm_frac_step": other["hbm_frac_step"],
+                          "ratio_split_over_fp64": (main_r["value"] / other["value"]
+                                                    if args.form == "split"
+                                                    else other["value"] / main_r["value"])}
+        out["split_error"] = err
     print(json.dumps(out), flush=True)
 
 

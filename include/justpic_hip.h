@@ -162,6 +162,39 @@ int jp_inject_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords
                         int min_xcell, uint64_t seed, uint64_t call_counter,
                         jp_stats* stats);
 
+/* ---- FP32-split form ---------------------------------------------------
+ * No reference counterpart: a build-only precision form of the same path.
+ * A position is the slot's cell (implicit) plus a float32 offset u in cell
+ * units per dimension; `off` are D float planes in the CellArray layout.
+ * Velocity interpolation and RK stages run in float32 in cell units; FP64 is
+ * used for the conversions below, for the offset renormalisation when a
+ * particle changes cell, and for the particle2grid weight sums.  Requires
+ * uniform vertex axes and the standard staggering (velocity component c on
+ * the vertex axis along c, on the centre axis with one ghost point per side
+ * elsewhere, README.md:42-44).  Cell assignment is integer: a particle stays
+ * in its cell while 0 <= u < 1 in every dimension. */
+int jp_split_from_f64(jp_ctx* ctx, const jp_cellgrid* g, const double* const* coords,
+                      const uint8_t* index, float* const* off);
+int jp_split_to_f64(jp_ctx* ctx, const jp_cellgrid* g, float* const* off,
+                    const uint8_t* index, double* const* coords);
+/* advection! on split positions (same methods as jp_advection) */
+int jp_advection_split(jp_ctx* ctx, const jp_cellgrid* g, float* const* off,
+                       const uint8_t* index, const double* const* V,
+                       const jp_velgrid* vg, int method, double alpha, double dt);
+/* move_particles! on split positions: same slot rules (colour phases,
+ * starting_point, drops) with integer cell assignment */
+int jp_move_particles_split(jp_ctx* ctx, const jp_cellgrid* g, float* const* off,
+                            uint8_t* index, int nargs, double* const* args,
+                            jp_stats* stats);
+int jp_particle2grid_split(jp_ctx* ctx, const jp_cellgrid* g, double* F,
+                           const double* Fp, float* const* off, const uint8_t* index);
+int jp_grid2particle_split(jp_ctx* ctx, const jp_cellgrid* g, double* Fp,
+                           const double* F, float* const* off, const uint8_t* index);
+int jp_inject_particles_split(jp_ctx* ctx, const jp_cellgrid* g, float* const* off,
+                              uint8_t* index, int nargs, double* const* args,
+                              int min_xcell, uint64_t seed, uint64_t call_counter,
+                              jp_stats* stats);
+
 /* Philox4x32-10 block (host helper, for tests): out[4] <- philox(ctr[4], key[2]) */
 void jp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

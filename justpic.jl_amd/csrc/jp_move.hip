@@ -24,6 +24,8 @@
 // loop for every phase in order (the other kernels return immediately).
 #include "jp_ctx.hpp"
 
+#include <type_traits>
+
 using namespace jp;
 
 namespace {
@@ -42,7 +44,8 @@ struct MoveScratch {
     uint64_t* tch;   // slots that received a particle at some point
     uint8_t* code;   // per slot (plane-major c + C*k): neighbour code / kDel, movers only
     uint32_t* org;   // per slot (plane-major): original slot index of an arrival
-    double* stg;     // (D + nargs) planes of C*K: staged mover data by original slot
+    void* stgc;      // D planes of C*K (coordinate type): staged mover coordinates
+    double* stga;    // nargs planes of C*K: staged mover fields, by original slot
 };
 
 template <int D>
@@ -144,8 +147,8 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                 cd = (uint8_t)n;
                 if (n != kCenter<D> && !in_cell_at<D>(g, J, p)) cd |= kGap;
 #pragma unroll
-                for (int d = 0; d < D; ++d) ms.stg[d * (C * g.K) + e] = p[d];
-                for (int a = 0; a < nargs; ++a) ms.stg[(D + a) * (C * g.K) + e] = args.p[a][e];
+                for (int d = 0; d < D; ++d) static_cast<double*>(ms.stgc)[d * (C * g.K) + e] = p[d];
+                for (int a = 0; a < nargs; ++a) ms.stga[a * (C * g.K) + e] = args.p[a][e];
             }
         }
         ms.code[e] = cd;  // plane-major, read back only for mover slots
@@ -299,8 +302,14 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
 }
 
 // ------------------------------------------------------------ 3. scatter
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, Ptr3 coords,
+// T: coordinate type (double: FP64 drop-in, float: split offsets)
+template <int D, typename T>
+struct TPtr3 {
+    T* p[3];
+};
+
+template <int D, typename T>
+__global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> coords,
                                                        uint8_t* __restrict__ index, int nargs,
                                                        ArgPtrs args, MoveScratch ms,
                                                        const int* __restrict__ flag) {
@@ -321,26 +330,28 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, Ptr3 coords,
     }
     const int64_t C = g.C, S = C * g.K;
     const int64_t e = c + C * k;
+    const T* __restrict__ sc = static_cast<const T*>(ms.stgc);
     if (arrived) {
         const int64_t o = ms.org[e];
 #pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = ms.stg[d * S + o];
-        for (int a = 0; a < nargs; ++a) args.p[a][e] = ms.stg[(D + a) * S + o];
+        for (int d = 0; d < D; ++d) coords.p[d][e] = sc[d * S + o];
+        for (int a = 0; a < nargs; ++a) args.p[a][e] = ms.stga[a * S + o];
         index[e] = 1;
     } else {  // vacated: empty_particle! (src/Particles/move_safe.jl:243-251)
 #pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
+        for (int d = 0; d < D; ++d) coords.p[d][e] = (T)dnan();
         for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
         index[e] = 0;
     }
 }
 
-// ------------------------------------------------------------ serial path
+// ------------------------------------------------------------ serial paths
 // move_kernel! (src/Particles/move_safe.jl:72-125) for one cell, directly on
 // the data (used when some particle jumps more than one cell).
 template <int D>
-__device__ void move_cell_serial(const Grid<D>& g, const Ptr3& coords, uint8_t* index, int nargs,
-                                 const ArgPtrs& args, const int64_t* I, int64_t (&cnt)[4]) {
+__device__ void move_cell_serial(const Grid<D>& g, const TPtr3<D, double>& coords, uint8_t* index,
+                                 int nargs, const ArgPtrs& args, const int64_t* I,
+                                 int64_t (&cnt)[4]) {
     const int64_t C = g.C;
     const int K = g.K;
     const int64_t c = lin<D>(g.ni, I);
@@ -395,9 +406,73 @@ __device__ void move_cell_serial(const Grid<D>& g, const Ptr3& coords, uint8_t* 
     }
 }
 
+// The same loop for split positions: the cell offset is floor(u); the
+// renormalised offset u - floor(u) moves with the particle.
 template <int D>
-__global__ void k_mv_serial(Grid<D> g, Ptr3 coords, uint8_t* index, int nargs, ArgPtrs args,
-                            const int* __restrict__ flag, jp_stats* st) {
+__device__ void move_cell_serial(const Grid<D>& g, const TPtr3<D, float>& off, uint8_t* index,
+                                 int nargs, const ArgPtrs& args, const int64_t* I,
+                                 int64_t (&cnt)[4]) {
+    const int64_t C = g.C;
+    const int K = g.K;
+    const int64_t c = lin<D>(g.ni, I);
+    int sp = 0;
+    for (int ip = 0; ip < K; ++ip) {
+        const int64_t e = c + C * ip;
+        if (!index[e]) continue;
+        float u[D];
+        int64_t J[D];
+        bool stay = true, out = false, far = false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            u[d] = off.p[d][e];
+            const bool fin = fabsf(u[d]) < 1.0e6f;  // false for NaN/Inf
+            const int s = fin ? (int)floorf(u[d]) : 0;
+            out = out || !fin;
+            J[d] = I[d] + s;
+            stay = stay && s == 0;
+            far = far || s > 1 || s < -1;
+            out = out || J[d] < 0 || J[d] >= g.ni[d];
+        }
+        if (stay && !out) continue;
+        if (out) {
+            index[e] = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) off.p[d][e] = __builtin_nanf("");
+            for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
+            cnt[1]++;
+            continue;
+        }
+        if (far) cnt[3]++;
+        double cached[JP_MAX_ARGS];
+        for (int a = 0; a < nargs; ++a) cached[a] = args.p[a][e];
+        index[e] = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) off.p[d][e] = __builtin_nanf("");
+        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
+        const int64_t cn = lin<D>(g.ni, J);
+        int fi = -1;
+        for (int i = sp; i < K; ++i)
+            if (!index[cn + C * i]) {
+                fi = i;
+                break;
+            }
+        if (fi < 0) {
+            cnt[2]++;
+            continue;
+        }
+        sp = fi;
+        const int64_t f = cn + C * fi;
+        index[f] = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) off.p[d][f] = (float)((double)u[d] - (double)(J[d] - I[d]));
+        for (int a = 0; a < nargs; ++a) args.p[a][f] = cached[a];
+        cnt[0]++;
+    }
+}
+
+template <int D, typename T>
+__global__ void k_mv_serial(Grid<D> g, TPtr3<D, T> coords, uint8_t* index, int nargs,
+                            ArgPtrs args, const int* __restrict__ flag, jp_stats* st) {
     if (!*flag || blockIdx.x != 0 || threadIdx.x != 0) return;
     int64_t ncol[D];
     int64_t total = 1;
@@ -431,26 +506,96 @@ __global__ void k_mv_serial(Grid<D> g, Ptr3 coords, uint8_t* index, int nargs, A
     stat_add(st, ST_FAR, cnt[3]);
 }
 
-Ptr3 ptr3(double* const* c, int D) {
-    Ptr3 p{{nullptr, nullptr, nullptr}};
-    for (int d = 0; d < D; ++d) p.p[d] = c[d];
-    return p;
+// ------------------------------------------------------------ split classify
+// Split positions: the slot's cell plus an offset u in cell units.  A particle
+// whose floor(u) is zero in every dimension stays; otherwise its destination
+// is the cell + floor(u) (deleted when that lies outside the grid), and the
+// staged offset is renormalised to the destination, u - floor(u), in FP64.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D, float> off,
+                                                              const uint8_t* __restrict__ index,
+                                                              int nargs, ArgPtrs args,
+                                                              MoveScratch ms, int* flag) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const int64_t C = g.C, S = C * g.K;
+    int64_t I[D];
+    unlin<D>(g.ni, c, I);
+    const uint64_t occ = active_slots(index, c, C, g.K);
+    uint64_t mov = 0;
+    float* __restrict__ sc = static_cast<float*>(ms.stgc);
+    constexpr int B = 8;
+    for (int k0 = 0; k0 < g.K; k0 += B) {
+        float ub[B][D];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const bool on = (occ >> (k0 + j)) & 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + C * (k0 + j)] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int k = k0 + j;
+            if (!((occ >> k) & 1)) continue;
+            int s[D];
+            bool stay = true, out = false, far = false;
+            int n = 0, mult = 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const bool fin = fabsf(ub[j][d]) < 1.0e6f;  // false for NaN/Inf
+                s[d] = fin ? (int)floorf(ub[j][d]) : 0;
+                out = out || !fin;
+                const int64_t Jd = I[d] + s[d];
+                out = out || Jd < 0 || Jd >= g.ni[d];
+                stay = stay && s[d] == 0;
+                far = far || s[d] > 1 || s[d] < -1;
+                n += (s[d] + 1) * mult;
+                mult *= 3;
+            }
+            if (stay && !out) continue;
+            mov |= 1ull << k;
+            const int64_t e = c + C * k;
+            uint8_t cd;
+            if (out) {
+                cd = kDel;
+            } else if (far) {
+                atomicOr(flag, 1);
+                cd = kDel;  // unused: the serial kernel takes over
+            } else {
+                cd = (uint8_t)n;
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+                    sc[d * S + e] = (float)((double)ub[j][d] - (double)s[d]);
+                for (int a = 0; a < nargs; ++a) ms.stga[a * S + e] = args.p[a][e];
+            }
+            ms.code[e] = cd;
+        }
+    }
+    ms.occ0[c] = occ;
+    ms.occ[c] = occ;
+    ms.mov[c] = mov;
+    ms.arr[c] = 0;
+    ms.tch[c] = 0;
 }
-CPtr3 cptr3(double* const* c, int D) {
-    CPtr3 p{{nullptr, nullptr, nullptr}};
-    for (int d = 0; d < D; ++d) p.p[d] = c[d];
-    return p;
-}
+
 ArgPtrs argptrs(double* const* a, int n) {
     ArgPtrs p;
     for (int i = 0; i < JP_MAX_ARGS; ++i) p.p[i] = i < n ? a[i] : nullptr;
     return p;
 }
 
+template <int D, typename T>
+TPtr3<D, T> tptr3(T* const* c) {
+    TPtr3<D, T> p{{nullptr, nullptr, nullptr}};
+    for (int d = 0; d < D; ++d) p.p[d] = c[d];
+    return p;
+}
+
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-template <int D>
-int run_move(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_t* index, int nargs,
+// classify -> 3^D colour phases -> serial fallback -> scatter
+template <int D, typename T>
+int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* index, int nargs,
              double* const* args, jp_stats* stats) {
     Grid<D> g = to_grid<D>(gc);
     const int64_t C = g.C, S = C * g.K;
@@ -465,7 +610,8 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_t*
     const size_t o_tch = off;  off = align_up(off + C * 8, 256);
     const size_t o_code = off; off = align_up(off + (size_t)S, 256);
     const size_t o_org = off;  off = align_up(off + (size_t)S * 4, 256);
-    const size_t o_stg = off;  off = align_up(off + (size_t)S * 8 * (D + nargs), 256);
+    const size_t o_stgc = off; off = align_up(off + (size_t)S * sizeof(T) * D, 256);
+    const size_t o_stga = off; off = align_up(off + (size_t)S * 8 * nargs, 256);
     int rc = ensure_scratch(ctx, off);
     if (rc) return rc;
     char* base = static_cast<char*>(ctx->scratch);
@@ -476,15 +622,24 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_t*
     ms.tch = (uint64_t*)(base + o_tch);
     ms.code = (uint8_t*)(base + o_code);
     ms.org = (uint32_t*)(base + o_org);
-    ms.stg = (double*)(base + o_stg);
+    ms.stgc = (void*)(base + o_stgc);
+    ms.stga = (double*)(base + o_stga);
 
     int* flag = dflags(ctx);
     if ((rc = zero_bytes(ctx, flag, sizeof(int)))) return rc;
     if ((rc = stats_begin(ctx, stats))) return rc;
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     ArgPtrs ap = argptrs(args, nargs);
-    k_mv_classify<D><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cptr3(coords, D), index, nargs,
-                                                               ap, ms, flag);
+    TPtr3<D, T> cp = tptr3<D, T>(coords);
+    if constexpr (std::is_same<T, double>::value) {
+        CPtr3 cc{{nullptr, nullptr, nullptr}};
+        for (int d = 0; d < D; ++d) cc.p[d] = coords[d];
+        k_mv_classify<D><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms,
+                                                                   flag);
+    } else {
+        k_mv_classify_split<D><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap,
+                                                                         ms, flag);
+    }
     JP_LAUNCHED(ctx);
     int64_t total = 1;
     for (int d = 0; d < D; ++d) total *= (g.ni[d] + 2) / 3;
@@ -501,10 +656,10 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_t*
                                                                                     flag, st);
         JP_LAUNCHED(ctx);
     }
-    k_mv_serial<D><<<1, 64, 0, ctx->stream>>>(g, ptr3(coords, D), index, nargs, ap, flag, st);
+    k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
     JP_LAUNCHED(ctx);
-    k_mv_scatter<D><<<dim3((unsigned)row_blocks<D>(g), (unsigned)g.K), kBlock, 0, ctx->stream>>>(
-        g, ptr3(coords, D), index, nargs, ap, ms, flag);
+    k_mv_scatter<D, T><<<dim3((unsigned)row_blocks<D>(g), (unsigned)g.K), kBlock, 0,
+                         ctx->stream>>>(g, cp, index, nargs, ap, ms, flag);
     JP_LAUNCHED(ctx);
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);
@@ -517,16 +672,28 @@ int check_ptrs(jp_ctx* ctx, int n, const void* const* p, const char* what) {
     return JP_OK;
 }
 
-}  // namespace
-
-extern "C" int jp_move_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
-                                 uint8_t* index, int nargs, double* const* args, jp_stats* stats) {
+template <typename T>
+int move_entry(jp_ctx* ctx, const jp_cellgrid* g, T* const* coords, uint8_t* index, int nargs,
+               double* const* args, jp_stats* stats) {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     if ((rc = check_ptrs(ctx, g->ndim, (const void* const*)coords, "coords"))) return rc;
     if (!index) return fail(ctx, JP_ERR_ARG, "null index%s");
     if (nargs < 0 || nargs > JP_MAX_ARGS) return fail(ctx, JP_ERR_ARG, "nargs out of range%s");
     if (nargs && (rc = check_ptrs(ctx, nargs, (const void* const*)args, "args"))) return rc;
-    return g->ndim == 2 ? run_move<2>(ctx, g, coords, index, nargs, args, stats)
-                        : run_move<3>(ctx, g, coords, index, nargs, args, stats);
+    return g->ndim == 2 ? run_move<2, T>(ctx, g, coords, index, nargs, args, stats)
+                        : run_move<3, T>(ctx, g, coords, index, nargs, args, stats);
+}
+
+}  // namespace
+
+extern "C" int jp_move_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                                 uint8_t* index, int nargs, double* const* args, jp_stats* stats) {
+    return move_entry<double>(ctx, g, coords, index, nargs, args, stats);
+}
+
+extern "C" int jp_move_particles_split(jp_ctx* ctx, const jp_cellgrid* g, float* const* off,
+                                       uint8_t* index, int nargs, double* const* args,
+                                       jp_stats* stats) {
+    return move_entry<float>(ctx, g, off, index, nargs, args, stats);
 }

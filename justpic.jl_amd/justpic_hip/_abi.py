@@ -107,6 +107,17 @@ _SIGS = {
     "jp_inject_particles": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
                                       C.c_int, C.c_uint64, C.c_uint64, C.POINTER(jp_stats)]),
     "jp_philox4x32_10": (None, [_P, _P, _P]),
+    "jp_split_from_f64": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P]),
+    "jp_split_to_f64": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P]),
+    "jp_advection_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P,
+                                     C.POINTER(jp_velgrid), C.c_int, C.c_double, C.c_double]),
+    "jp_move_particles_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
+                                          C.POINTER(jp_stats)]),
+    "jp_particle2grid_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P]),
+    "jp_grid2particle_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P]),
+    "jp_inject_particles_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
+                                            C.c_int, C.c_uint64, C.c_uint64,
+                                            C.POINTER(jp_stats)]),
 }
 
 

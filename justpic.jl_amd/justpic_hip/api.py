@@ -1,5 +1,7 @@
 """The reference's public hot-path API on ``Particles`` (trailing ``_`` marks
-Julia's ``!``).  Each function performs exactly one C-ABI call."""
+Julia's ``!``).  Each function performs exactly one C-ABI call (tuples of
+fields on split containers: one per field); containers in the FP32-split form
+(``particles.form == "split"``) go to the ``*_split`` entry points."""
 from __future__ import annotations
 
 import ctypes as C
@@ -55,7 +57,8 @@ def advection_(particles: Particles, method: AbstractAdvectionIntegrator, V, dt)
         if tuple(V[c].shape) != want:
             raise ValueError(f"V[{c}] has shape {V[c].shape}, expected {want}")
     ctx = particles.ctx
-    ctx.check(ctx.lib.jp_advection(ctx.h, particles.grid_ref(), particles.coord_ptrs(),
+    fn = ctx.lib.jp_advection_split if particles.form == "split" else ctx.lib.jp_advection
+    ctx.check(fn(ctx.h, particles.grid_ref(), particles.coord_ptrs(),
                                    particles.index.ptr, _ptrs(V), particles.velgrid_ref(),
                                    method.code, method.alpha, float(dt)), "advection!")
 
@@ -65,7 +68,9 @@ def move_particles_(particles: Particles, args=(), stats: bool = False):
     Returns the outcome counters when ``stats`` (costs one host sync)."""
     ctx = particles.ctx
     st = _abi.jp_stats() if stats else None
-    ctx.check(ctx.lib.jp_move_particles(ctx.h, particles.grid_ref(), particles.coord_ptrs(),
+    fn = (ctx.lib.jp_move_particles_split if particles.form == "split"
+          else ctx.lib.jp_move_particles)
+    ctx.check(fn(ctx.h, particles.grid_ref(), particles.coord_ptrs(),
                                         particles.index.ptr, len(args), _ptrs(args),
                                         C.byref(st) if st is not None else None),
               "move_particles!")
@@ -82,6 +87,12 @@ def particle2grid_(F, Fp, particles: Particles):
     """``particle2grid!(F, Fp, particles)`` (src/Interpolations/particle_to_grid.jl:23-28);
     F/Fp may be tuples (tuple methods :70-109, :153-199)."""
     ctx = particles.ctx
+    if isinstance(F, (tuple, list)) and particles.form == "split":
+        if len(F) != len(Fp):
+            raise ValueError("F and Fp tuples differ in length")
+        for f, fp in zip(F, Fp):
+            particle2grid_(f, fp, particles)
+        return
     if isinstance(F, (tuple, list)):
         if len(F) != len(Fp):
             raise ValueError("F and Fp tuples differ in length")
@@ -92,7 +103,9 @@ def particle2grid_(F, Fp, particles: Particles):
                                                  particles.index.ptr), "particle2grid!")
     else:
         _check_nodal(F, particles)
-        ctx.check(ctx.lib.jp_particle2grid(ctx.h, particles.grid_ref(), F.ptr, Fp.ptr,
+        fn = (ctx.lib.jp_particle2grid_split if particles.form == "split"
+              else ctx.lib.jp_particle2grid)
+        ctx.check(fn(ctx.h, particles.grid_ref(), F.ptr, Fp.ptr,
                                            particles.coord_ptrs(), particles.index.ptr),
                   "particle2grid!")
 
@@ -105,7 +118,9 @@ def grid2particle_(Fp, F, particles: Particles):
             grid2particle_(fp, f, particles)
         return
     _check_nodal(F, particles)
-    ctx.check(ctx.lib.jp_grid2particle(ctx.h, particles.grid_ref(), Fp.ptr, F.ptr,
+    fn = (ctx.lib.jp_grid2particle_split if particles.form == "split"
+          else ctx.lib.jp_grid2particle)
+    ctx.check(fn(ctx.h, particles.grid_ref(), Fp.ptr, F.ptr,
                                        particles.coord_ptrs(), particles.index.ptr),
               "grid2particle!")
 
@@ -115,7 +130,9 @@ def inject_particles_(particles: Particles, args=(), stats: bool = False):
     Each call advances ``particles.inject_counter`` (the Philox counter)."""
     ctx = particles.ctx
     st = _abi.jp_stats() if stats else None
-    ctx.check(ctx.lib.jp_inject_particles(ctx.h, particles.grid_ref(), particles.coord_ptrs(),
+    fn = (ctx.lib.jp_inject_particles_split if particles.form == "split"
+          else ctx.lib.jp_inject_particles)
+    ctx.check(fn(ctx.h, particles.grid_ref(), particles.coord_ptrs(),
                                           particles.index.ptr, len(args), _ptrs(args),
                                           particles.min_xcell, particles.seed,
                                           particles.inject_counter,

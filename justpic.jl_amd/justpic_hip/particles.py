@@ -148,6 +148,64 @@ class Particles:
         return ([c.data.flat() for c in self.coords], self.index.data.flat())
 
 
+class SplitParticles(Particles):
+    """Particles in the FP32-split form (no reference counterpart).
+
+    ``coords`` hold D float32 CellArrays of offsets u in cell units: slot
+    (c, k) of a cell with first vertex x_I and spacing h sits at
+    ``x_I + u*h``.  Built from an FP64 container by :func:`to_split`, converted
+    back by :func:`to_f64`; the hot-path functions of :mod:`.api` accept both
+    forms.  Requires uniform vertex axes and the standard velocity staggering.
+    """
+
+    form = "split"
+
+    @property
+    def off(self):
+        return self.coords
+
+
+Particles.form = "f64"
+
+
+def _check_uniform(p: Particles):
+    for d, a in enumerate(p.xvi):
+        if a.dx.size and not np.all(a.dx == a.dx[0]):
+            raise ValueError(f"FP32-split form needs a uniform vertex axis (axis {d} is not)")
+
+
+def to_split(p: Particles) -> SplitParticles:
+    """FP32-split copy of an FP64 container (same index, seed and counter)."""
+    if p.form != "f64":
+        raise ValueError("to_split expects an FP64 Particles container")
+    _check_uniform(p)
+    ctx = p.ctx
+    off = [CellArray(ctx, p.ni, p.max_xcell, dtype=np.float32) for _ in range(p.D)]
+    index = p.index.copy()
+    s = SplitParticles(ctx, off, index, p.nxcell, p.max_xcell, p.min_xcell, p.xvi, p.xci,
+                       p.xi_vel, seed=p.seed)
+    s.inject_counter = p.inject_counter
+    ctx.check(ctx.lib.jp_split_from_f64(ctx.h, s.grid_ref(), p.coord_ptrs(), index.ptr,
+                                        s.coord_ptrs()), "to_split")
+    return s
+
+
+def to_f64(s: SplitParticles) -> Particles:
+    """FP64 container with the absolute coordinates of a split container
+    (inactive slots NaN, as everywhere else)."""
+    if s.form != "split":
+        raise ValueError("to_f64 expects a SplitParticles container")
+    ctx = s.ctx
+    coords = [CellArray(ctx, s.ni, s.max_xcell) for _ in range(s.D)]
+    index = s.index.copy()
+    p = Particles(ctx, coords, index, s.nxcell, s.max_xcell, s.min_xcell, s.xvi, s.xci,
+                  s.xi_vel, seed=s.seed)
+    p.inject_counter = s.inject_counter
+    ctx.check(ctx.lib.jp_split_to_f64(ctx.h, s.grid_ref(), s.coord_ptrs(), index.ptr,
+                                      p.coord_ptrs()), "to_f64")
+    return p
+
+
 def _split_grids(xi_vel):
     D = len(xi_vel)
     xvi = tuple(xi_vel[i][i] for i in range(D))
@@ -175,10 +233,20 @@ def particles_from_host(ctx: Context, xi_vel, coords, index, nxcell, max_xcell, 
     return Particles(ctx, cs, idx, nxcell, max_xcell, min_xcell, xvi, xci, xi_vel, seed=seed)
 
 
-def init_particles(ctx: Context, nxcell, max_xcell, min_xcell, *xi_vel, seed: int = 0):
+def init_particles(ctx: Context, nxcell, max_xcell, min_xcell, *xi_vel, seed: int = 0,
+                   form: str = "f64"):
     """``init_particles(backend, nxcell, max_xcell, min_xcell, xi_vel...)``
     (src/Particles/particles_utils.jl:30-166): nxcell rounded up to a multiple
-    of 2^D, K = max(nxcell, max_xcell), quadrant-stratified random fill."""
+    of 2^D, K = max(nxcell, max_xcell), quadrant-stratified random fill.
+    ``form="split"`` returns the FP32-split form of the same fill."""
+    if form not in ("f64", "split"):
+        raise ValueError("form must be 'f64' or 'split'")
+    if form == "split":
+        p = init_particles(ctx, nxcell, max_xcell, min_xcell, *xi_vel, seed=seed)
+        s = to_split(p)
+        for c in p.coords:
+            c.data.free()
+        return s
     if len(xi_vel) == 1 and isinstance(xi_vel[0], tuple) and xi_vel and \
             isinstance(xi_vel[0][0], tuple):
         xi_vel = xi_vel[0]
