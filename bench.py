@@ -79,7 +79,8 @@ def run_form(jp, ctx, cfg, form, args):
     call is bracketed by hipEvents on the context stream."""
     from justpic_hip.configs import gaussian_T
     xv, xi, V, dt = cfg["xv"], cfg["xi_vel"], cfg["V"], cfg["dt"]
-    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed, form=form)
+    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed,
+                          form="split" if form == "split" else "f64")
     Vd = tuple(jp.DeviceArray.from_host(ctx, v) for v in V)
     T = jp.DeviceArray.from_host(ctx, gaussian_T(xv))
     pT = jp.CellArray(ctx, p.ni, p.max_xcell, fill=float("nan"))

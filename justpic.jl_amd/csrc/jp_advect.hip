@@ -1,12 +1,10 @@
 // jp_advect.hip -- advection! (src/Particles/Advection/advection.jl:35-91) on
 // gfx950, FP64 drop-in, 2D and 3D, Euler / RK2 / RK4.
 //
-// One thread per slot: a wavefront owns 64 consecutive cells along x of one
-// grid row and one slot plane, so its mask and coordinate accesses are
-// contiguous in the cell-plane layout.  The velocity arrays are small and stay
-// in L1/L2; a bracket lookup costs one guess (uniform spacing estimate) and one
-// strict test against the materialised axis, with the reference search for
-// everything else (ties, refined axes).
+// One workgroup per tile of cells with the tile's velocity and axis windows
+// in LDS (see the tiled kernel below); a bracket lookup costs one guess
+// (uniform spacing estimate) and one strict test against the stored axis, with
+// the reference search for everything else (ties, refined axes).
 #include "jp_ctx.hpp"
 
 using namespace jp;
@@ -26,66 +24,147 @@ struct RKParams {
     double w1;   // 0.5*inv(alpha)
 };
 
-// Unique strict bracket of p on axis `a` when the guess hits it, else the
-// reference search seeded at the particle's cell (same result either way: a
-// strictly interior point has one bracket whatever the seed).
-__device__ inline int64_t bracket(double p, const Axis& a, int64_t seed) {
-    const int64_t last = a.n - 2;
-    int64_t g = (int64_t)((p - a.x[0]) * a.idx[0]);
-    g = g < 0 ? 0 : (g > last ? last : g);
-    if (a.x[g] < p && p < a.x[g + 1]) return g;
-    return find_cell(p, a, seed);
-}
-
 // interp_velocity2particle (advection.jl:93-148) for component c: +Inf outside
 // the component's grid (check_local_limits, src/Advection/advection.jl:83-91),
 // t = (p - x_k)*inv(dx_k) (normalize_coordinates, src/Interpolations/utils.jl:54-58),
 // n-linear lerp of the corner values (src/Interpolations/ndlerp.jl:11-15) in
 // extract_field_corners order (src/Advection/advection.jl:47-71).
+
+// ---------------------------------------------------------------- tiled kernel
+// One workgroup per tile of cells (2D 64 x 4, 3D 16 x 4 x 4), one thread per
+// cell, slots walked in chunks of 8 planes in lockstep across the wavefront.
+// Per tile, LDS holds for every velocity component c the corner values of the
+// node window [t0 - 1, t0 + T + 3) per dimension, and for every axis (c, d) the
+// coordinates and inverse spacings of the same node window.  A bracket is the
+// uniform-spacing guess confirmed by the strict test x[k] < p < x[k+1] on the
+// stored coordinates (a strictly interior point has one bracket, the one the
+// reference's bisection returns whatever its seed); ties and misses take the
+// seeded bisection on the global axis.  Values read from LDS are the stored
+// values, so results are identical to the global-memory path.
 template <int D>
-__device__ inline double interp_c(const Axis* ax, const double* __restrict__ F,
-                                  const double (&p)[D], const int64_t (&I)[D]) {
+__host__ __device__ constexpr int tile_n(int d) {
+    return D == 2 ? (d == 0 ? 64 : 4) : (d == 0 ? 16 : 4);
+}
+template <int D>
+__host__ __device__ constexpr int win_n(int d) {
+    return tile_n<D>(d) + 4;
+}
+template <int D>
+__host__ __device__ constexpr int win_size() {
+    return D == 2 ? win_n<2>(0) * win_n<2>(1) : win_n<3>(0) * win_n<3>(1) * win_n<3>(2);
+}
+template <int D>
+__host__ __device__ constexpr int win_max() {
+    return win_n<D>(0);
+}
+
+template <int D>
+struct TileLDS {
+    double V[D][win_size<D>()];
+    double x[D][D][win_max<D>()];    // axis (c, d): x[k], k = t0[d] - 1 + local
+    double idx[D][D][win_max<D>()];  // 1 / dx[k]
+    double lim[D][D][2];             // first and last coordinate of axis (c, d)
+    double g0[D][D][2];              // x[0] and inv_dx[0]: the uniform guess
+};
+
+// ties and guesses outside the window: the seeded search on the global axis
+__device__ inline void bracket_global(double p, const Axis& A, int64_t seed, int64_t& k,
+                                           double& xk, double& ik) {
+    k = find_cell(p, A, seed);
+    xk = A.x[k];
+    ik = A.idx[k];
+}
+
+// bracket of p on axis (c, d): global node k, its coordinate and inverse spacing
+template <int D>
+__device__ inline void bracket_tile(const TileLDS<D>& L, const Axis& A, int c, int d, int base,
+                                    double p, int64_t seed, int64_t& k, double& xk, double& ik,
+                                    bool& inw) {
+    const int64_t last = A.n - 2;
+    int64_t gk = (int64_t)((p - L.g0[c][d][0]) * L.g0[c][d][1]);
+    gk = gk < 0 ? 0 : (gk > last ? last : gk);
+    const int64_t kl = gk - base;
+    if (kl >= 0 && kl + 1 < win_n<D>(d)) {
+        const double a = L.x[c][d][kl], b = L.x[c][d][kl + 1];
+        if (a < p && p < b) {
+            k = gk;
+            xk = a;
+            ik = L.idx[c][d][kl];
+            inw = true;
+            return;
+        }
+    }
+    bracket_global(p, A, seed, k, xk, ik);
+    const int64_t kk = k - base;
+    inw = kk >= 0 && kk + 1 < win_n<D>(d);
+}
+
+template <int D>
+__device__ inline double interp_tile(const TileLDS<D>& L, const Vel<D>& vg, const VPtrs<D>& V,
+                                     int c, const int (&base)[D], const double (&p)[D],
+                                     const int64_t (&I)[D]) {
     bool in = true;
 #pragma unroll
-    for (int d = 0; d < D; ++d) in = in && (ax[d].x[0] <= p[d] && p[d] <= ax[d].x[ax[d].n - 1]);
+    for (int d = 0; d < D; ++d) in = in && (L.lim[c][d][0] <= p[d] && p[d] <= L.lim[c][d][1]);
     if (!in) return __builtin_inf();
     int64_t k[D];
     double t[D];
+    bool inw = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        k[d] = bracket(p[d], ax[d], I[d]);
-        t[d] = (p[d] - ax[d].x[k[d]]) * ax[d].idx[k[d]];
+        double xk, ik;
+        bool w;
+        bracket_tile<D>(L, vg.ax[c][d], c, d, base[d], p[d], I[d], k[d], xk, ik, w);
+        t[d] = (p[d] - xk) * ik;
+        inw = inw && w;
     }
-    const int64_t n0 = ax[0].n;
-    if constexpr (D == 2) {
-        const int64_t b = k[0] + n0 * k[1];
-        return lerp1(t[1], lerp1(t[0], F[b], F[b + 1]), lerp1(t[0], F[b + n0], F[b + n0 + 1]));
+    double q[1 << D];
+    if (inw) {
+        constexpr int W0 = win_n<D>(0), W1 = win_n<D>(1);
+        int b = (int)(k[0] - base[0]) + W0 * (int)(k[1] - base[1]);
+        if constexpr (D == 3) b += W0 * W1 * (int)(k[2] - base[2]);
+#pragma unroll
+        for (int r = 0; r < (1 << D); ++r) {
+            const int o = (r & 1) + ((r >> 1) & 1) * W0 + (D == 3 ? ((r >> 2) & 1) * W0 * W1 : 0);
+            q[r] = L.V[c][b + o];
+        }
     } else {
-        const int64_t n1 = ax[1].n, s = n0 * n1;
-        const int64_t b = k[0] + n0 * (k[1] + n1 * k[2]);
-        const double lo =
-            lerp1(t[1], lerp1(t[0], F[b], F[b + 1]), lerp1(t[0], F[b + n0], F[b + n0 + 1]));
-        const double hi = lerp1(t[1], lerp1(t[0], F[b + s], F[b + s + 1]),
-                                lerp1(t[0], F[b + s + n0], F[b + s + n0 + 1]));
+        const double* __restrict__ F = V.v[c];
+        const int64_t n0 = vg.ax[c][0].n, n1 = vg.ax[c][1].n;
+        int64_t b = k[0] + n0 * k[1];
+        if constexpr (D == 3) b += n0 * n1 * k[2];
+#pragma unroll
+        for (int r = 0; r < (1 << D); ++r) {
+            const int64_t o = (r & 1) + ((r >> 1) & 1) * n0 + (D == 3 ? ((r >> 2) & 1) * n0 * n1 : 0);
+            q[r] = F[b + o];
+        }
+    }
+    const double lo = lerp1(t[1], lerp1(t[0], q[0], q[1]), lerp1(t[0], q[2], q[3]));
+    if constexpr (D == 2) {
+        return lo;
+    } else {
+        const double hi = lerp1(t[1], lerp1(t[0], q[4], q[5]), lerp1(t[0], q[6], q[7]));
         return lerp1(t[2], lo, hi);
     }
 }
 
 template <int D>
-__device__ inline void interp_v(const Vel<D>& vg, const VPtrs<D>& V, const double (&p)[D],
-                                const int64_t (&I)[D], double (&v)[D]) {
+__device__ inline void interp_v_tile(const TileLDS<D>& L, const Vel<D>& vg, const VPtrs<D>& V,
+                                     const int (&base)[D], const double (&p)[D],
+                                     const int64_t (&I)[D], double (&v)[D]) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) v[c] = interp_c<D>(vg.ax[c], V.v[c], p, I);
+    for (int c = 0; c < D; ++c) v[c] = interp_tile<D>(L, vg, V, c, base, p, I);
 }
 
-// advect_particle: Euler (Euler.jl:1-20), RK2 (RK2.jl:1-26, stages
-// src/Advection/RK2.jl:1-44), RK4 (RK4.jl:1-19, no muladd)
+// advect_particle with the tile's LDS: Euler (Euler.jl:1-20), RK2 (RK2.jl:1-26,
+// stages src/Advection/RK2.jl:1-44), RK4 (RK4.jl:1-19, no muladd)
 template <int D, int M>
-__device__ inline void advect_one(const Vel<D>& vg, const VPtrs<D>& V, const RKParams& P,
-                                  const double (&p0)[D], const int64_t (&I)[D],
-                                  double (&out)[D]) {
+__device__ inline void advect_tile(const TileLDS<D>& L, const Vel<D>& vg, const VPtrs<D>& V,
+                                   const RKParams& P, const int (&base)[D],
+                                   const double (&p0)[D], const int64_t (&I)[D],
+                                   double (&out)[D]) {
     double v0[D];
-    interp_v<D>(vg, V, p0, I, v0);
+    interp_v_tile<D>(L, vg, V, base, p0, I, v0);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v0[d], p0[d]);
@@ -93,7 +172,7 @@ __device__ inline void advect_one(const Vel<D>& vg, const VPtrs<D>& V, const RKP
         double p1[D], v1[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) p1[d] = fma(P.adt, v0[d], p0[d]);
-        interp_v<D>(vg, V, p1, I, v1);  // same seed I (RK2.jl:14,20)
+        interp_v_tile<D>(L, vg, V, base, p1, I, v1);  // same seed I (RK2.jl:14,20)
         if (P.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v1[d], p0[d]);
@@ -105,13 +184,13 @@ __device__ inline void advect_one(const Vel<D>& vg, const VPtrs<D>& V, const RKP
         double q[D], k2[D], k3[D], k4[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * v0[d]) / 2.0;
-        interp_v<D>(vg, V, q, I, k2);
+        interp_v_tile<D>(L, vg, V, base, q, I, k2);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * k2[d]) / 2.0;
-        interp_v<D>(vg, V, q, I, k3);
+        interp_v_tile<D>(L, vg, V, base, q, I, k3);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + P.dt * k3[d];
-        interp_v<D>(vg, V, q, I, k4);
+        interp_v_tile<D>(L, vg, V, base, q, I, k4);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const double sum = ((v0[d] + 2.0 * k2[d]) + 2.0 * k3[d]) + k4[d];
@@ -124,16 +203,101 @@ __device__ inline void advect_one(const Vel<D>& vg, const VPtrs<D>& V, const RKP
 template <int D, int M>
 __global__ void __launch_bounds__(kBlock) k_advect(Grid<D> g, Vel<D> vg, VPtrs<D> V, Ptr3 coords,
                                                    const uint8_t* __restrict__ index, RKParams P) {
-    int64_t c, I[D];
-    if (!row_cell<D>(g, c, I)) return;
-    const int64_t e = c + g.C * (int64_t)blockIdx.y;
-    if (!index[e]) return;
-    double p0[D], p[D];
+    __shared__ TileLDS<D> L;
+    int t0[D], base[D];
+    {
+        int64_t b = blockIdx.x;
 #pragma unroll
-    for (int d = 0; d < D; ++d) p0[d] = coords.p[d][e];
-    advect_one<D, M>(vg, V, P, p0, I, p);
+        for (int d = 0; d < D; ++d) {
+            const int64_t nt = (g.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
+            t0[d] = (int)(b % nt) * tile_n<D>(d);
+            b /= nt;
+            base[d] = t0[d] - 1;
+        }
+    }
+    const int tid = threadIdx.x;
 #pragma unroll
-    for (int d = 0; d < D; ++d) coords.p[d][e] = p[d];
+    for (int c = 0; c < D; ++c) {
+        const double* __restrict__ F = V.v[c];
+        for (int i = tid; i < win_size<D>(); i += kBlock) {
+            int r = i;
+            int64_t node = 0, stride = 1;
+            bool ok = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int64_t k = base[d] + r % win_n<D>(d);
+                r /= win_n<D>(d);
+                ok = ok && k >= 0 && k < vg.ax[c][d].n;
+                node += k * stride;
+                stride *= vg.ax[c][d].n;
+            }
+            L.V[c][i] = ok ? F[node] : 0.0;
+        }
+    }
+    constexpr int NA = D * D * win_max<D>();
+    for (int i = tid; i < NA; i += kBlock) {
+        const int c = i / (D * win_max<D>());
+        const int d = (i / win_max<D>()) % D;
+        const int j = i % win_max<D>();
+        const Axis& A = vg.ax[c][d];
+        const int64_t k = base[d] + j;
+        const bool okx = j < win_n<D>(d) && k >= 0 && k < A.n;
+        const bool oki = j < win_n<D>(d) && k >= 0 && k < A.n - 1;
+        L.x[c][d][j] = okx ? A.x[k] : __builtin_nan("");
+        L.idx[c][d][j] = oki ? A.idx[k] : __builtin_nan("");
+    }
+    if (tid < D * D) {
+        const int c = tid / D, d = tid % D;
+        const Axis& A = vg.ax[c][d];
+        L.lim[c][d][0] = A.x[0];
+        L.lim[c][d][1] = A.x[A.n - 1];
+        L.g0[c][d][0] = A.x[0];
+        L.g0[c][d][1] = A.idx[0];
+    }
+    __syncthreads();
+    int64_t I[D];
+    bool in = true;
+    {
+        int r = tid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            I[d] = t0[d] + r % tile_n<D>(d);
+            r /= tile_n<D>(d);
+            in = in && I[d] < g.ni[d];
+        }
+    }
+    if (!in) return;
+    const int64_t c = lin<D>(g.ni, I);
+    const int64_t C = g.C;
+    const uint64_t act = active_slots(index, c, C, g.K);
+    constexpr int B = 8;
+    for (int k0 = 0; k0 < g.K; k0 += B) {
+        const uint32_t ab = (uint32_t)(act >> k0) & 0xFFu;
+        if (!ab) continue;
+        double pb[B][D];
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                pb[j][d] = ((ab >> j) & 1) ? coords.p[d][c + C * (k0 + j)] : 0.0;
+        // one particle at a time (the chunk's loads are already in flight)
+        uint32_t todo = ab;
+        while (todo) {
+            const int j = __builtin_ctz(todo);
+            todo &= todo - 1;
+            double p0[D], out[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double v = pb[0][d];
+#pragma unroll
+                for (int q = 1; q < B; ++q) v = j == q ? pb[q][d] : v;
+                p0[d] = v;
+            }
+            advect_tile<D, M>(L, vg, V, P, base, p0, I, out);
+#pragma unroll
+            for (int d = 0; d < D; ++d) coords.p[d][c + C * (k0 + j)] = out[d];
+        }
+    }
 }
 
 template <int D>
@@ -150,8 +314,9 @@ Vel<D> to_vel(const jp_velgrid* vg) {
 template <int D, int M>
 int launch(jp_ctx* ctx, const Grid<D>& G, const Vel<D>& vg, const VPtrs<D>& V, Ptr3 cp,
            const uint8_t* index, const RKParams& P) {
-    dim3 grid((unsigned)row_blocks<D>(G), (unsigned)G.K);
-    k_advect<D, M><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P);
+    int64_t tiles = 1;
+    for (int d = 0; d < D; ++d) tiles *= (G.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
+    k_advect<D, M><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P);
     JP_LAUNCHED(ctx);
     return JP_OK;
 }

@@ -184,6 +184,25 @@ __device__ inline uint64_t active_slots(const uint8_t* __restrict__ index, int64
     return bits;
 }
 
+// Appends c to the list segment of its wavefront and colour without atomics:
+// segment (q, wave) holds up to 64 cells at seg[(q*nwave + wave)*64 + i] and
+// its length at cnt[q*nwave + wave]; a compaction kernel joins the segments.
+// Every lane of the wavefront must call it (want = false: nothing to add).
+template <int NQ>
+__device__ inline void wave_segment(bool want, int col, int32_t c, int64_t wave, int64_t nwave,
+                                    int32_t* __restrict__ seg, int32_t* __restrict__ cnt) {
+    if (wave >= nwave) return;  // a whole wavefront past the last cell
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const bool mine = want && col == q;
+        const uint64_t m = __ballot(mine);
+        if (mine) seg[(q * nwave + wave) * 64 + __popcll(m & below)] = c;
+        if (lane == 0) cnt[q * nwave + wave] = __popcll(m);
+    }
+}
+
 // Iterates the set bits of `bits` in ascending order, loading NV values per
 // slot (from planes p[0..NV-1], element c + C*k) two slots ahead of use.
 template <int NV>

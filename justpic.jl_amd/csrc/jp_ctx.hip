@@ -17,6 +17,47 @@ __global__ void k_fill_f64(double* p, double v, int64_t n) {
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+// 64-bit occupancy word of every cell (bit k = mask of slot k)
+__global__ void __launch_bounds__(kBlock) k_occ_words(const uint8_t* __restrict__ index, int64_t C,
+                                                      int K, uint64_t* __restrict__ occ) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    occ[c] = active_slots(index, c, C, K);
+}
+
+// Joins the per-wavefront list segments of one colour (blockIdx.x) into
+// list[q*cap + i], in wavefront order, and writes the colour's total.
+__global__ void __launch_bounds__(1024) k_join_segments(const int32_t* __restrict__ seg,
+                                                        const int32_t* __restrict__ cnt,
+                                                        int64_t nwave, int32_t* __restrict__ list,
+                                                        int32_t* __restrict__ total, int64_t cap) {
+    __shared__ int32_t sh[1024];
+    const int q = blockIdx.x, t = threadIdx.x;
+    const int64_t per = (nwave + 1023) / 1024;
+    const int64_t lo = t * per < nwave ? t * per : nwave;
+    const int64_t hi = lo + per < nwave ? lo + per : nwave;
+    const int32_t* __restrict__ cq = cnt + q * nwave;
+    int32_t s = 0;
+    for (int64_t w = lo; w < hi; ++w) s += cq[w];
+    sh[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan
+        const int32_t v = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    int32_t off = sh[t] - s;
+    if (t == 1023) total[q] = sh[t];
+    int32_t* __restrict__ lq = list + q * cap;
+    for (int64_t w = lo; w < hi; ++w) {
+        const int32_t n = cq[w];
+        const int32_t* __restrict__ sw = seg + (q * nwave + w) * 64;
+        for (int i = 0; i < n; ++i) lq[off + i] = sw[i];
+        off += n;
+    }
+}
+
 __global__ void k_fill_f32(float* p, float v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -45,6 +86,36 @@ int ensure_scratch(jp_ctx* ctx, size_t bytes) {
     size_t sz = (bytes + 4095) & ~(size_t)4095;
     JP_HIP(ctx, hipMalloc(&ctx->scratch, sz));
     ctx->scratch_bytes = sz;
+    return JP_OK;
+}
+
+int occ_words(jp_ctx* ctx, const uint8_t* index, int64_t C, int K, uint64_t* occ) {
+    k_occ_words<<<blocks_for(C), kBlock, 0, ctx->stream>>>(index, C, K, occ);
+    JP_LAUNCHED(ctx);
+    return JP_OK;
+}
+
+int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, InjectLists& L) {
+    // scratch: totals | counts | segments | lists
+    L.nwave = (C + 63) / 64;
+    const size_t a = 256;
+    const size_t b = a + (((size_t)nq * L.nwave * 4 + 255) & ~(size_t)255);
+    const size_t c = b + (size_t)nq * L.nwave * 64 * 4;
+    const size_t d = c + (size_t)nq * cap * 4;
+    int rc = ensure_scratch(ctx, d);
+    if (rc) return rc;
+    char* base = static_cast<char*>(ctx->scratch);
+    L.total = reinterpret_cast<int32_t*>(base);
+    L.cnt = reinterpret_cast<int32_t*>(base + a);
+    L.seg = reinterpret_cast<int32_t*>(base + b);
+    L.list = reinterpret_cast<int32_t*>(base + c);
+    L.cap = cap;
+    return JP_OK;
+}
+
+int join_segments(jp_ctx* ctx, int nq, const InjectLists& L) {
+    k_join_segments<<<nq, 1024, 0, ctx->stream>>>(L.seg, L.cnt, L.nwave, L.list, L.total, L.cap);
+    JP_LAUNCHED(ctx);
     return JP_OK;
 }
 
@@ -133,11 +204,11 @@ int jp_ctx_create(jp_ctx** out, int device) {
 
 int jp_ctx_destroy(jp_ctx* ctx) {
     if (!ctx) return JP_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->own) hipStreamSynchronize(ctx->own);
-    if (ctx->scratch) hipFree(ctx->scratch);
-    if (ctx->dsmall) hipFree(ctx->dsmall);
-    if (ctx->own) hipStreamDestroy(ctx->own);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->dsmall) (void)hipFree(ctx->dsmall);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return JP_OK;
 }
@@ -271,7 +342,7 @@ int jp_graph_end(jp_ctx* ctx, void** exec) {
     JP_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
     hipGraphExec_t ge;
     hipError_t e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     JP_HIP(ctx, e);
     *exec = ge;
     return JP_OK;

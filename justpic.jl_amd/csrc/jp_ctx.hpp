@@ -51,6 +51,25 @@ inline int* dflags(jp_ctx* ctx) {
 
 // scratch of at least `bytes`; grows outside capture only
 int ensure_scratch(jp_ctx* ctx, size_t bytes);
+// Injection work lists: per-wavefront segments from the scan, joined per
+// colour into list[q*cap + i] with total[q] entries (no atomics).
+struct InjectLists {
+    int32_t* total;
+    int32_t* cnt;
+    int32_t* seg;
+    int32_t* list;
+    int64_t nwave;
+    int64_t cap;
+};
+int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, InjectLists& L);
+int join_segments(jp_ctx* ctx, int nq, const InjectLists& L);
+// persistent wavefronts for a work list of at most n entries
+inline int64_t inject_blocks(int64_t n) {
+    const int64_t w = n < 4096 ? n : 4096;
+    return (w + 3) / 4 > 0 ? (w + 3) / 4 : 1;
+}
+// occupancy words (bit k = slot k active) of the C cells, one kernel
+int occ_words(jp_ctx* ctx, const uint8_t* index, int64_t C, int K, uint64_t* occ);
 // zero `n` ints/bytes with a kernel (no memset nodes inside graphs)
 int zero_bytes(jp_ctx* ctx, void* p, size_t bytes);
 // stats bracket: zero the device stats before, copy + accumulate after

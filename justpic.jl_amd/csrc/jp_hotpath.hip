@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) k_init(Grid<D> g, Ptr3 coords, uint8_t
 // outermost, slots ascending).  mode 0: wF/w (2D scalar), 1: wF*inv(w).
 template <int D, int NF>
 __global__ void __launch_bounds__(kBlock) k_p2g(Grid<D> g, int nf_rt, ArgPtrs F, ArgPtrs Fp,
-                                                CPtr3 coords, const uint8_t* __restrict__ index,
+                                                CPtr3 coords, const uint64_t* __restrict__ occw,
                                                 int mode) {
     // NF == 1: one field, accumulator kept in a register; NF == 0: nf_rt fields
     const int nf = NF == 1 ? 1 : nf_rt;
@@ -86,8 +86,10 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Grid<D> g, int nf_rt, ArgPtrs F,
     for (int f = 0; f < nf; ++f) wF[f] = 0.0;
     const int64_t C = g.C;
     constexpr int NC = 1 << D;
-    // the (up to) 2^D cells around the node, last dim outermost
+    // the (up to) 2^D cells around the node, last dim outermost, and their
+    // occupancy words (one round of loads)
     int64_t cell[NC];
+    uint64_t bits[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         int64_t I[D];
@@ -97,60 +99,73 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Grid<D> g, int nf_rt, ArgPtrs F,
             I[d] = N[d] + (((q >> d) & 1) ? 0 : -1);
             ok = ok && (I[d] >= 0 && I[d] < g.ni[d]);
         }
-        cell[q] = ok ? lin<D>(g.ni, I) : -1;
+        cell[q] = ok ? lin<D>(g.ni, I) : 0;
+        bits[q] = ok ? occw[cell[q]] : 0ull;
     }
-    // their mask bytes, all cells' chunk loads in flight together
-    uint64_t bits[NC];
+    // Reference order: cells, then slots ascending.  The (cell, chunk of B
+    // slots) sequence is walked with the next chunk's loads in flight while
+    // the current one is accumulated; all lanes of a wavefront are on the same
+    // slot plane, so every load is coalesced across neighbouring cells.
+    constexpr int B = D == 2 ? 8 : 4;  // two chunk buffers stay in registers
+    const int nk = (g.K + B - 1) / B;
+    const int total = NC * nk;
+    auto chunk = [&](int t, int64_t& c, uint32_t& bq, int& k0) __attribute__((always_inline)) {
+        const int q = t / nk;
+        k0 = (t - q * nk) * B;
+        c = cell[0];
+        uint64_t bw = bits[0];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) bits[q] = 0;
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
-        uint8_t m[NC][8];
-#pragma unroll
-        for (int q = 0; q < NC; ++q)
-#pragma unroll
-            for (int v = 0; v < 8; ++v)
-                m[q][v] = (cell[q] >= 0 && k0 + v < g.K) ? index[cell[q] + C * (k0 + v)] : (uint8_t)0;
-#pragma unroll
-        for (int q = 0; q < NC; ++q)
-#pragma unroll
-            for (int v = 0; v < 8; ++v) bits[q] |= (uint64_t)(m[q][v] != 0) << (k0 + v);
-    }
-    // reference order (cells, then slots ascending).  All lanes of a
-    // wavefront walk the same slot plane at the same time, so every load is
-    // coalesced across the wavefront's neighbouring cells; a chunk of slots is
-    // loaded (predicated on the mask bit) before it is accumulated.
-    constexpr int B = D == 2 ? 16 : 8;
-#pragma unroll 1
-    for (int q = 0; q < NC; ++q) {
-        const int64_t c = cell[q] < 0 ? 0 : cell[q];
-        const uint64_t bq = bits[q];
-        for (int k0 = 0; k0 < g.K; k0 += B) {
-            double pv[B][D + 1];
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                const bool on = (bq >> (k0 + j)) & 1;  // bits above K are 0
-                const int64_t e = c + C * (k0 + j);
-#pragma unroll
-                for (int d = 0; d < D; ++d) pv[j][d] = on ? coords.p[d][e] : 0.0;
-                pv[j][D] = on ? Fp.p[0][e] : 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                if (!((bq >> (k0 + j)) & 1)) continue;
-                double a = xn[0] - pv[j][0];
-                double s = a * a;
-#pragma unroll
-                for (int d = 1; d < D; ++d) {
-                    double b = xn[d] - pv[j][d];
-                    s = s + b * b;
-                }
-                const double dist = sqrt(s);
-                const double wi = 1.0 / (dist * dist);
-                w += wi;
-                wF[0] = fma(wi, pv[j][D], wF[0]);
-                for (int f = 1; f < nf; ++f) wF[f] = fma(wi, Fp.p[f][c + C * (k0 + j)], wF[f]);
-            }
+        for (int u = 1; u < NC; ++u) {
+            c = q == u ? cell[u] : c;
+            bw = q == u ? bits[u] : bw;
         }
+        bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
+    };
+    auto load = [&](int t, double (&pv)[B][D + 1]) __attribute__((always_inline)) {
+        int64_t c;
+        uint32_t bq;
+        int k0;
+        chunk(t, c, bq, k0);
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const bool on = (bq >> j) & 1;
+            const int64_t e = c + C * (k0 + j);
+#pragma unroll
+            for (int d = 0; d < D; ++d) pv[j][d] = on ? coords.p[d][e] : 0.0;
+            pv[j][D] = on ? Fp.p[0][e] : 0.0;
+        }
+    };
+    auto accumulate = [&](int t, const double (&pv)[B][D + 1]) __attribute__((always_inline)) {
+        int64_t c;
+        uint32_t bq;
+        int k0;
+        chunk(t, c, bq, k0);
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (!((bq >> j) & 1)) continue;
+            double a = xn[0] - pv[j][0];
+            double s = a * a;
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                double b = xn[d] - pv[j][d];
+                s = s + b * b;
+            }
+            const double dist = sqrt(s);
+            const double wi = 1.0 / (dist * dist);
+            w += wi;
+            wF[0] = fma(wi, pv[j][D], wF[0]);
+            for (int f = 1; f < nf; ++f) wF[f] = fma(wi, Fp.p[f][c + C * (k0 + j)], wF[f]);
+        }
+    };
+    double b0[B][D + 1], b1[B][D + 1];
+    load(0, b0);
+#pragma unroll 1
+    for (int t = 0; t < total; t += 2) {
+        if (t + 1 < total) load(t + 1, b1);
+        accumulate(t, b0);
+        if (t + 1 >= total) break;
+        if (t + 2 < total) load(t + 2, b0);
+        accumulate(t + 1, b1);
     }
     if (mode == 0) {
         F.p[0][node] = wF[0] / w;
@@ -199,51 +214,50 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Grid<D> g, double* __restrict__ 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
                                                         const uint8_t* __restrict__ index,
-                                                        int min_q, int32_t* __restrict__ lists,
-                                                        int32_t* __restrict__ counts,
-                                                        int64_t cap) {
+                                                        int min_q, InjectLists L) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= g.C) return;
+    const bool valid = c < g.C;  // all lanes stay for the wavefront append
     int64_t I[D];
-    unlin<D>(g.ni, c, I);
-    double lo[D], hq[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        lo[d] = g.xv[d].x[I[d]];
-        hq[d] = g.xv[d].dx[I[d]] / 2.0;
-    }
-    // quadrant c11: vertex + dq*0 (src/Particles/injection.jl:442-448)
-    double q0[D], q1[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        q0[d] = lo[d] + hq[d] * 0.0;
-        q1[d] = q0[d] + hq[d];
-    }
+    unlin<D>(g.ni, valid ? c : 0, I);
     int num = 0;
-    const uint64_t act = active_slots(index, c, g.C, g.K);
-    constexpr int B = 16;  // slots per chunk; lanes share the slot plane (coalesced)
-    for (int k0 = 0; k0 < g.K; k0 += B) {
-        double pb[B][D];
+    if (valid) {
+        double lo[D], hq[D];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            const bool on = (act >> (k0 + j)) & 1;
-#pragma unroll
-            for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + g.C * (k0 + j)] : 0.0;
+        for (int d = 0; d < D; ++d) {
+            lo[d] = g.xv[d].x[I[d]];
+            hq[d] = g.xv[d].dx[I[d]] / 2.0;
         }
+        // quadrant c11: vertex + dq*0 (src/Particles/injection.jl:442-448)
+        double q0[D], q1[D];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            bool inside = (act >> (k0 + j)) & 1;
+        for (int d = 0; d < D; ++d) {
+            q0[d] = lo[d] + hq[d] * 0.0;
+            q1[d] = q0[d] + hq[d];
+        }
+        const uint64_t act = active_slots(index, c, g.C, g.K);
+        constexpr int B = 16;  // slots per chunk; lanes share the slot plane (coalesced)
+        for (int k0 = 0; k0 < g.K; k0 += B) {
+            double pb[B][D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) inside = inside & (q0[d] < pb[j][d] && pb[j][d] < q1[d]);
-            num += inside ? 1 : 0;
+            for (int j = 0; j < B; ++j) {
+                const bool on = (act >> (k0 + j)) & 1;
+#pragma unroll
+                for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + g.C * (k0 + j)] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                bool inside = (act >> (k0 + j)) & 1;
+#pragma unroll
+                for (int d = 0; d < D; ++d) inside = inside & (q0[d] < pb[j][d] && pb[j][d] < q1[d]);
+                num += inside ? 1 : 0;
+            }
         }
     }
-    if (num >= min_q) return;
     int col = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) col = col * 2 + (int)(I[d] & 1);
-    const int32_t slot = atomicAdd(&counts[col], 1);
-    lists[col * cap + slot] = (int32_t)c;
+    wave_segment<D == 2 ? 4 : 8>(valid && num < min_q, col, (int32_t)c, c >> 6, L.nwave, L.seg,
+                                 L.cnt);
 }
 
 // _inject_particles! (src/Particles/injection.jl:68-131) with one wavefront per
@@ -261,8 +275,13 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
                                                         const int32_t* __restrict__ count,
                                                         jp_stats* st) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6);
-    if (w >= *count) return;  // uniform across the wavefront
+    // persistent wavefronts over the colour's list (every wavefront exits once
+    // the list is exhausted)
+    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t total = *count;
+    int64_t n_inj = 0, n_orph = 0;
+    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
+         w += nwv) {
     const int64_t c = list[w];
     const int64_t C = g.C;
     const int K = g.K;
@@ -285,7 +304,6 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
     double p[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) p[d] = act ? coords.p[d][e] : 0.0;
-    int64_t n_inj = 0, n_orph = 0;
     for (int q = 0; q < NQ; ++q) {
         double vert[D];
 #pragma unroll
@@ -338,17 +356,22 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
                         ok = ok && J[d] >= 0 && J[d] < g.ni[d];
                     }
                     e2[u] = ok ? lin<D>(g.ni, J) + C * lane : 0;
-                    if (nb == CENTER)
-                        a[u] = act && lane != ip;
-                    else
-                        a[u] = ok && index[e2[u]] != 0;
+                    a[u] = ok;
                 }
+                // masks and coordinates of the batch in one round of loads
+                // (an empty slot holds NaN coordinates and is masked out below)
+                uint8_t m[NBB];
                 double pq[NBB][D];
 #pragma unroll
-                for (int u = 0; u < NBB; ++u)
+                for (int u = 0; u < NBB; ++u) {
+                    const bool other = a[u] && nb0 + u != CENTER;
+                    m[u] = other ? index[e2[u]] : (uint8_t)0;
 #pragma unroll
-                    for (int d = 0; d < D; ++d)
-                        pq[u][d] = (nb0 + u == CENTER) ? p[d] : (a[u] ? coords.p[d][e2[u]] : 0.0);
+                    for (int d = 0; d < D; ++d) pq[u][d] = other ? coords.p[d][e2[u]] : p[d];
+                }
+#pragma unroll
+                for (int u = 0; u < NBB; ++u)
+                    a[u] = (nb0 + u == CENTER) ? (act && lane != ip) : (m[u] != 0);
 #pragma unroll
                 for (int u = 0; u < NBB; ++u) {
                     if (!a[u]) continue;
@@ -402,6 +425,130 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
         }
         (void)done;
     }
+    }  // work list
+    if (lane == 0) {
+        stat_add(st, ST_INJECTED, n_inj);
+        stat_add(st, ST_ORPHAN, n_orph);
+    }
+}
+
+// The same injection for 2D with at most one field, the neighbourhood held in
+// registers: each lane loads its slot of the 3x3 cells (mask, coordinates
+// and the field) once per listed cell; nearest-neighbour copies then come
+// from registers through a shuffle, so an injection issues only stores.
+__global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords, uint8_t* index,
+                                                         int nargs, double* __restrict__ arg,
+                                                         int min_q, uint64_t seed,
+                                                         uint64_t counter,
+                                                         const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ count,
+                                                         jp_stats* st) {
+    constexpr int NB = 9, CENTER = 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t total = *count;
+    const int64_t C = g.C;
+    const int K = g.K;
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    const bool own = lane < K;
+    int64_t n_inj = 0, n_orph = 0;
+    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
+         w += nwv) {
+        const int64_t c = __builtin_amdgcn_readfirstlane(list[w]);
+        int64_t I[2];
+        unlin<2>(g.ni, c, I);
+        double xv[2], dq[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            xv[d] = g.xv[d].x[I[d]];
+            dq[d] = g.xv[d].dx[I[d]] / 2.0;
+        }
+        // lane's slot of the 3x3 neighbourhood, one round of loads
+        uint8_t m[NB];
+        double q[NB][2], a[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
+            const bool ok = own && J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            const int64_t e2 = J0 + g.ni[0] * J1 + C * lane;
+            m[nb] = ok ? index[e2] : (uint8_t)0;
+            q[nb][0] = ok ? coords.p[0][e2] : 0.0;
+            q[nb][1] = ok ? coords.p[1][e2] : 0.0;
+            a[nb] = (ok && nargs) ? arg[e2] : 0.0;
+        }
+        bool act = m[CENTER] != 0;
+        for (int qd = 0; qd < 4; ++qd) {
+            double vert[2];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) vert[d] = xv[d] + dq[d] * (double)((qd >> d) & 1);
+            bool inside = act;
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+                inside = inside & (vert[d] < q[CENTER][d] && q[CENTER][d] < vert[d] + dq[d]);
+            int num = __popcll(__ballot(inside));
+            if (num >= min_q) break;
+            uint64_t freeS = ~__ballot(act) & full;
+            while (freeS) {
+                const int ip = __builtin_ctzll(freeS);
+                freeS &= freeS - 1;
+                ++num;
+                double pn[2];
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    const double r = uniform53(seed, (uint64_t)c, ip, d, 2, counter);
+                    pn[d] = vert[d] + dq[d] * fma(0.95, r, 0.05);
+                }
+                const int64_t enew = c + C * ip;
+                if (lane == ip) {
+                    act = true;
+                    q[CENTER][0] = pn[0];
+                    q[CENTER][1] = pn[1];
+                    coords.p[0][enew] = pn[0];
+                    coords.p[1][enew] = pn[1];
+                    index[enew] = 1;
+                }
+                // index_min_distance: first occurrence in (cell, slot) order
+                double best = __builtin_inf();
+                int border = 0x7fffffff;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const bool av = nb == CENTER ? (act && lane != ip) : (m[nb] != 0);
+                    if (!av) continue;
+                    if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
+                    const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
+                    const double dd = sqrt(du * du + dv * dv);
+                    if (dd < best) {
+                        best = dd;
+                        border = nb * 64 + lane;
+                    }
+                }
+#pragma unroll
+                for (int s = 32; s >= 1; s >>= 1) {
+                    const double ob = __shfl_xor(best, s, 64);
+                    const int oo = __shfl_xor(border, s, 64);
+                    if (ob < best || (ob == best && oo < border)) {
+                        best = ob;
+                        border = oo;
+                    }
+                }
+                if (border == 0x7fffffff) {
+                    ++n_orph;  // no neighbour at all: the field is left as it was
+                } else if (nargs) {
+                    const int nbw = border >> 6;  // uniform
+                    double mine = a[0];
+#pragma unroll
+                    for (int nb = 1; nb < NB; ++nb) mine = nb == nbw ? a[nb] : mine;
+                    const double v = __shfl(mine, border & 63, 64);
+                    if (lane == ip) {
+                        a[CENTER] = v;
+                        arg[enew] = v;
+                    }
+                }
+                ++n_inj;
+                if (num >= min_q) break;
+            }
+        }
+    }  // work list
     if (lane == 0) {
         stat_add(st, ST_INJECTED, n_inj);
         stat_add(st, ST_ORPHAN, n_orph);
@@ -463,12 +610,16 @@ int run_p2g(jp_ctx* ctx, const jp_cellgrid* g, int nf, double* const* F, const d
     for (int d = 0; d < D; ++d) NN *= G.ni[d] + 1;
     ArgPtrs Fa = argptrs(F, nf);
     ArgPtrs Fpa = argptrs(const_cast<double* const*>(Fp), nf);
+    int rc;
+    if ((rc = ensure_scratch(ctx, (size_t)G.C * 8))) return rc;
+    uint64_t* occ = static_cast<uint64_t*>(ctx->scratch);
+    if ((rc = occ_words(ctx, index, G.C, G.K, occ))) return rc;
     if (nf == 1)
         k_p2g<D, 1><<<blocks_for(NN), kBlock, 0, ctx->stream>>>(G, nf, Fa, Fpa, cptr3(coords, D),
-                                                                index, mode);
+                                                                occ, mode);
     else
         k_p2g<D, 0><<<blocks_for(NN), kBlock, 0, ctx->stream>>>(G, nf, Fa, Fpa, cptr3(coords, D),
-                                                                index, mode);
+                                                                occ, mode);
     JP_LAUNCHED(ctx);
     return finish(ctx);
 }
@@ -495,23 +646,30 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
     // cells per colour: at most prod(ceil(ni/2))
     int64_t cap = 1;
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
-    const size_t counts_bytes = 256;
-    if ((rc = ensure_scratch(ctx, counts_bytes + (size_t)NQ * cap * 4))) return rc;
-    int32_t* counts = static_cast<int32_t*>(ctx->scratch);
-    int32_t* lists = reinterpret_cast<int32_t*>(static_cast<char*>(ctx->scratch) + counts_bytes);
-    if ((rc = zero_bytes(ctx, counts, counts_bytes))) return rc;
+    InjectLists L;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, L))) return rc;
     k_inject_scan<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D), index,
-                                                                  min_q, lists, counts, cap);
+                                                                  min_q, L);
     JP_LAUNCHED(ctx);
+    if ((rc = join_segments(ctx, NQ, L))) return rc;
     Ptr3 cp = ptr3(coords, D);
     ArgPtrs ap = argptrs(args, nargs);
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     for (int ph = 0; ph < NQ; ++ph) {
         // colour index of the phase: offsets (o_1 outer ... o_D inner)
-        // one wavefront per listed cell (kBlock/64 cells per block)
-        k_inject_wave<D><<<blocks_for(cap, kBlock / 64), kBlock, 0, ctx->stream>>>(
-            G, cp, index, nargs, ap, min_q, seed, counter, lists + (int64_t)ph * cap,
-            counts + ph, st);
+        // persistent wavefronts, one listed cell at a time each
+        if constexpr (D == 2) {
+            if (nargs <= 1) {
+                k_inject_wave2<<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+                    G, cp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
+                    L.list + (int64_t)ph * cap, L.total + ph, st);
+                JP_LAUNCHED(ctx);
+                continue;
+            }
+        }
+        k_inject_wave<D><<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+            G, cp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
+            L.total + ph, st);
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;

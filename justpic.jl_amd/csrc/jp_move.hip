@@ -30,7 +30,6 @@ using namespace jp;
 
 namespace {
 
-constexpr uint8_t kStay = 0xFF;
 constexpr uint8_t kDel = 0xFE;
 constexpr uint8_t kGap = 0x80;  // arrival fails the strict in-cell test of its destination
 
@@ -42,7 +41,7 @@ struct MoveScratch {
     uint64_t* mov;   // slots holding a particle that moves when its cell's phase runs
     uint64_t* arr;   // slots holding a particle that arrived during this call
     uint64_t* tch;   // slots that received a particle at some point
-    uint8_t* code;   // per slot (plane-major c + C*k): neighbour code / kDel, movers only
+    uint8_t* code;   // per slot, 8 per word (word j*C + c): neighbour code / kDel, movers only
     uint32_t* org;   // per slot (plane-major): original slot index of an arrival
     void* stgc;      // D planes of C*K (coordinate type): staged mover coordinates
     double* stga;    // nargs planes of C*K: staged mover fields, by original slot
@@ -82,6 +81,26 @@ __device__ inline int phase_of(const int64_t* I) {
 }
 
 // ------------------------------------------------------------ 1. classify
+// Fields of the movers of one chunk of 8 slots, copied to the staging area in
+// one batch of independent loads (stg: chunk-local mover bits).
+__device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint32_t stg,
+                                  int nargs, const ArgPtrs& args, const MoveScratch& ms) {
+    if (!stg) return;
+    for (int a = 0; a < nargs; ++a) {
+        const double* __restrict__ src = args.p[a];
+        double* __restrict__ dst = ms.stga + a * S;
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = ((stg >> j) & 1) ? src[c + C * (k0 + j)] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if ((stg >> j) & 1) dst[c + C * (k0 + j)] = v[j];
+    }
+}
+
+// Destination codes are packed 8 per 64-bit word, word j of cell c at word
+// index j*C + c (coalesced stores here), so the colour phases load a cell's
+// codes in one round of word loads.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                                                         const uint8_t* __restrict__ index,
@@ -89,70 +108,99 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                                                         int* flag) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= g.C) return;
-    const int64_t C = g.C;
+    const int64_t C = g.C, S = C * g.K;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
-    const uint64_t occ = active_slots(index, c, C, g.K);
-    uint64_t mov = 0;
-    // cell geometry (isincell uses the scalar/vector vertex spacing)
-    double lo[D], hi[D];
+    // vertices x[I-1..I+2] and spacings dx[I-1..I+1] per dimension, loaded once:
+    // the strict brackets of the three candidate cells are register compares
+    // (a strictly interior point has a unique bracket, so this equals the
+    // bisection seeded at the cell; ties and longer jumps take find_cell)
+    double xa[D][4], hx[D][3], dlo[D], dhi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        lo[d] = g.xv[d].x[I[d]];
-        hi[d] = lo[d] + g.xv[d].dx[I[d]];
+        const Axis& A = g.xv[d];
+        const int64_t i = I[d], n = A.n;
+        xa[d][0] = i > 0 ? A.x[i - 1] : -__builtin_inf();
+        xa[d][1] = A.x[i];
+        xa[d][2] = A.x[i + 1];
+        xa[d][3] = i + 2 < n ? A.x[i + 2] : __builtin_inf();
+        hx[d][0] = i > 0 ? A.dx[i - 1] : 0.0;
+        hx[d][1] = A.dx[i];
+        hx[d][2] = i + 1 < n - 1 ? A.dx[i + 1] : 0.0;
+        dlo[d] = A.x[0];
+        dhi[d] = A.x[n - 1];
     }
+    const uint64_t occ = active_slots(index, c, C, g.K);
+    uint64_t mov = 0;
+    double* __restrict__ sc = static_cast<double*>(ms.stgc);
+    uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
     // loads and stores across neighbouring cells)
     constexpr int B = 8;
     for (int k0 = 0; k0 < g.K; k0 += B) {
-    double pb[B][D];
+        double pb[B][D];
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-        const bool on = (occ >> (k0 + j)) & 1;
+        for (int j = 0; j < B; ++j) {
+            const bool on = (occ >> (k0 + j)) & 1;
 #pragma unroll
-        for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + C * (k0 + j)] : 0.0;
-    }
+            for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + C * (k0 + j)] : 0.0;
+        }
+        uint64_t word = 0;
+        uint32_t mv8 = 0, stg = 0;
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-        const int k = k0 + j;
-        if (!((occ >> k) & 1)) continue;
-        double p[D];
+        for (int j = 0; j < B; ++j) {
+            const int k = k0 + j;
+            if (!((occ >> k) & 1)) continue;
+            double p[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = pb[j][d];
-        bool inc = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) inc = inc & (lo[d] < p[d] && p[d] < hi[d]);
-        if (inc) continue;
-        mov |= 1ull << k;
-        const int64_t e = c + C * k;
-        uint8_t cd;
-        if (!in_domain<D>(g, p)) {
-            cd = kDel;
-        } else {
-            int n = 0, mult = 1;
-            bool far = false;
-            int64_t J[D];
+            for (int d = 0; d < D; ++d) p[d] = pb[j][d];
+            bool inc = true, indom = true;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                J[d] = find_cell(p[d], g.xv[d], I[d]);
-                int64_t dl = J[d] - I[d];
-                far = far || dl > 1 || dl < -1;
-                n += (int)(dl + 1) * mult;
-                mult *= 3;
+                inc = inc & (xa[d][1] < p[d] && p[d] < xa[d][1] + hx[d][1]);  // isincell
+                indom = indom & (dlo[d] < p[d] && p[d] < dhi[d]);  // indomain
             }
-            if (far) {
-                atomicOr(flag, 1);
-                cd = kDel;  // unused: the serial kernel takes over
+            if (inc) continue;
+            mv8 |= 1u << j;
+            uint8_t cd;
+            if (!indom) {
+                cd = kDel;
             } else {
-                cd = (uint8_t)n;
-                if (n != kCenter<D> && !in_cell_at<D>(g, J, p)) cd |= kGap;
+                int n = 0, mult = 1;
+                bool far = false, inJ = true;
 #pragma unroll
-                for (int d = 0; d < D; ++d) static_cast<double*>(ms.stgc)[d * (C * g.K) + e] = p[d];
-                for (int a = 0; a < nargs; ++a) ms.stga[a * (C * g.K) + e] = args.p[a][e];
+                for (int d = 0; d < D; ++d) {
+                    int r;
+                    if (xa[d][0] < p[d] && p[d] < xa[d][1]) r = -1;
+                    else if (xa[d][1] < p[d] && p[d] < xa[d][2]) r = 0;
+                    else if (xa[d][2] < p[d] && p[d] < xa[d][3]) r = 1;
+                    else {
+                        const int64_t t = find_cell(p[d], g.xv[d], I[d]) - I[d];
+                        r = t < -1 ? -2 : (t > 1 ? 2 : (int)t);
+                    }
+                    far = far || r < -1 || r > 1;
+                    n += (r + 1) * mult;
+                    mult *= 3;
+                    const double lo = r < 0 ? xa[d][0] : (r == 0 ? xa[d][1] : xa[d][2]);
+                    const double h = r < 0 ? hx[d][0] : (r == 0 ? hx[d][1] : hx[d][2]);
+                    inJ = inJ & (lo < p[d] && p[d] < lo + h);
+                }
+                if (far) {
+                    atomicOr(flag, 1);
+                    cd = kDel;  // unused: the serial kernel takes over
+                } else {
+                    cd = (uint8_t)n;
+                    if (n != kCenter<D> && !inJ) cd |= kGap;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) sc[d * S + c + C * k] = p[d];
+                    stg |= 1u << j;
+                }
             }
+            word |= (uint64_t)cd << (8 * j);
         }
-        ms.code[e] = cd;  // plane-major, read back only for mover slots
-    }
+        if (mv8) cw[(k0 >> 3) * C] = word;
+        mov |= (uint64_t)mv8 << k0;
+        stage_args(c, C, S, k0, stg, nargs, args, ms);
     }
     ms.occ0[c] = occ;
     ms.occ[c] = occ;
@@ -207,15 +255,21 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     uint64_t pend = ms.mov[S];
     uint64_t arrS = ms.arr[S];
     uint64_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // movers' code bytes, 8 per word
+    if (pend) {
+        const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S;
+        const int nw = (K + 7) >> 3;
 #pragma unroll
-    for (int j = 0; j < 64; ++j)
-        if (j < K && ((pend >> j) & 1)) cw[j >> 3] |= (uint64_t)ms.code[S + C * j] << (8 * (j & 7));
+        for (int j = 0; j < 8; ++j)
+            if (j < nw) cw[j] = cp[j * C];
+    }
     int64_t nbc[kNB<D>];
+    uint64_t was[kNB<D>];
 #pragma unroll
     for (int n = 0; n < kNB<D>; ++n) {
         bool v;
         nbc[n] = nb_cell<D>(g, I, n, v);
-        s_occ[n][tid] = v ? ms.occ[nbc[n]] : ~0ull;  // outside the grid: never free
+        was[n] = v ? ms.occ[nbc[n]] : ~0ull;  // outside the grid: never free
+        s_occ[n][tid] = was[n];
     }
     if (!pend) return;  // nothing leaves or re-slots in this cell
     uint64_t selfm = 0;  // slots that received a re-slotted particle of this cell
@@ -276,7 +330,7 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
             unlin<D>(g.ni, Dc, J);
             if (phase_of<D>(J) > phase) {
                 ms.mov[Dc] |= 1ull << f;
-                ms.code[Dc + C * f] = (uint8_t)kCenter<D>;
+                ms.code[((f >> 3) * C + Dc) * 8 + (f & 7)] = (uint8_t)kCenter<D>;
             }
         }
     }
@@ -284,9 +338,8 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     for (int n = 0; n < kNB<D>; ++n) {
         if (n == kCenter<D> || nbc[n] < 0) continue;
         const uint64_t now = s_occ[n][tid];
-        const uint64_t was = ms.occ[nbc[n]];
-        if (now != was) {
-            const uint64_t ins = now & ~was;
+        if (now != was[n]) {
+            const uint64_t ins = now & ~was[n];
             ms.occ[nbc[n]] = now;
             ms.arr[nbc[n]] |= ins;
             ms.tch[nbc[n]] |= ins;
@@ -313,35 +366,47 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
                                                        uint8_t* __restrict__ index, int nargs,
                                                        ArgPtrs args, MoveScratch ms,
                                                        const int* __restrict__ flag) {
-    // one thread per slot; a wavefront covers 64 neighbouring cells of one
-    // slot plane (blockIdx.y), so the per-cell words, origins and the final
-    // stores are coalesced; only the staged-data reads are scattered.
+    // one thread per cell, chunks of 8 slot planes in lockstep across the
+    // wavefront (coalesced origin loads and final stores); only the reads of
+    // the staged data are scattered.  Arrivals copy their staged data, vacated
+    // slots are emptied (empty_particle!, src/Particles/move_safe.jl:243-251).
     if (*flag) return;
-    int64_t c, I[D];
-    if (!row_cell<D>(g, c, I)) return;
-    const int k = blockIdx.y;
-    const uint64_t bit = 1ull << k;
-    const uint64_t arr = ms.arr[c];
-    const bool arrived = arr & bit;
-    if (!arrived) {
-        const uint64_t occF = ms.occ[c];
-        if (occF & bit) return;  // untouched particle
-        if (!((ms.occ0[c] | ms.tch[c]) & bit)) return;  // was and stays empty
-    }
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= g.C) return;
+    const uint64_t arr = ms.arr[c], occ = ms.occ[c], occ0 = ms.occ0[c], tch = ms.tch[c];
+    const uint64_t work = arr | (~arr & ~occ & (occ0 | tch));
+    if (!work) return;
     const int64_t C = g.C, S = C * g.K;
-    const int64_t e = c + C * k;
     const T* __restrict__ sc = static_cast<const T*>(ms.stgc);
-    if (arrived) {
-        const int64_t o = ms.org[e];
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+        const uint32_t wb = (uint32_t)(work >> k0) & 0xFFu;
+        if (!wb) continue;
+        const uint32_t ab = (uint32_t)(arr >> k0) & 0xFFu;
+        uint32_t o[8];
 #pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = sc[d * S + o];
-        for (int a = 0; a < nargs; ++a) args.p[a][e] = ms.stga[a * S + o];
-        index[e] = 1;
-    } else {  // vacated: empty_particle! (src/Particles/move_safe.jl:243-251)
+        for (int j = 0; j < 8; ++j) o[j] = ((ab >> j) & 1) ? ms.org[c + C * (k0 + j)] : 0u;
+        T v[8][D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = (T)dnan();
-        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-        index[e] = 0;
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                v[j][d] = ((ab >> j) & 1) ? sc[d * S + o[j]] : (T)dnan();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (!((wb >> j) & 1)) continue;
+            const int64_t e = c + C * (k0 + j);
+#pragma unroll
+            for (int d = 0; d < D; ++d) coords.p[d][e] = v[j][d];
+            index[e] = (uint8_t)((ab >> j) & 1);
+        }
+        for (int a = 0; a < nargs; ++a) {
+            double av[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) av[j] = ((ab >> j) & 1) ? ms.stga[a * S + o[j]] : dnan();
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if ((wb >> j) & 1) args.p[a][c + C * (k0 + j)] = av[j];
+        }
     }
 }
 
@@ -524,6 +589,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
     const uint64_t occ = active_slots(index, c, C, g.K);
     uint64_t mov = 0;
     float* __restrict__ sc = static_cast<float*>(ms.stgc);
+    uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     constexpr int B = 8;
     for (int k0 = 0; k0 < g.K; k0 += B) {
         float ub[B][D];
@@ -533,6 +599,8 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
 #pragma unroll
             for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + C * (k0 + j)] : 0.0f;
         }
+        uint64_t word = 0;
+        uint32_t mv8 = 0, stg = 0;
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const int k = k0 + j;
@@ -553,8 +621,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 mult *= 3;
             }
             if (stay && !out) continue;
-            mov |= 1ull << k;
-            const int64_t e = c + C * k;
+            mv8 |= 1u << j;
             uint8_t cd;
             if (out) {
                 cd = kDel;
@@ -565,11 +632,14 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 cd = (uint8_t)n;
 #pragma unroll
                 for (int d = 0; d < D; ++d)
-                    sc[d * S + e] = (float)((double)ub[j][d] - (double)s[d]);
-                for (int a = 0; a < nargs; ++a) ms.stga[a * S + e] = args.p[a][e];
+                    sc[d * S + c + C * k] = (float)((double)ub[j][d] - (double)s[d]);
+                stg |= 1u << j;
             }
-            ms.code[e] = cd;
+            word |= (uint64_t)cd << (8 * j);
         }
+        if (mv8) cw[(k0 >> 3) * C] = word;
+        mov |= (uint64_t)mv8 << k0;
+        stage_args(c, C, S, k0, stg, nargs, args, ms);
     }
     ms.occ0[c] = occ;
     ms.occ[c] = occ;
@@ -608,7 +678,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     const size_t o_mov = off;  off = align_up(off + C * 8, 256);
     const size_t o_arr = off;  off = align_up(off + C * 8, 256);
     const size_t o_tch = off;  off = align_up(off + C * 8, 256);
-    const size_t o_code = off; off = align_up(off + (size_t)S, 256);
+    const size_t o_code = off; off = align_up(off + (size_t)C * 64, 256);
     const size_t o_org = off;  off = align_up(off + (size_t)S * 4, 256);
     const size_t o_stgc = off; off = align_up(off + (size_t)S * sizeof(T) * D, 256);
     const size_t o_stga = off; off = align_up(off + (size_t)S * 8 * nargs, 256);
@@ -658,8 +728,8 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     }
     k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
     JP_LAUNCHED(ctx);
-    k_mv_scatter<D, T><<<dim3((unsigned)row_blocks<D>(g), (unsigned)g.K), kBlock, 0,
-                         ctx->stream>>>(g, cp, index, nargs, ap, ms, flag);
+    k_mv_scatter<D, T><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
+                                                                 flag);
     JP_LAUNCHED(ctx);
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);

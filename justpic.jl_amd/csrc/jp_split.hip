@@ -19,13 +19,6 @@ struct FPtr3 {
     float* p[3];
 };
 
-// scaled velocity: V'_c = V_c * dt / h_c (cell displacement per step), float
-template <int D>
-struct SVel {
-    const float* v[D];
-    int n[D][D];  // dims of component c's grid
-};
-
 template <int D>
 struct SGrid {
     int ni[D];
@@ -87,21 +80,44 @@ __global__ void __launch_bounds__(kBlock) k_from_split(Grid<D> g, FPtr3 u, const
         x.p[d][e] = g.xv[d].x[I[d]] + (double)u.p[d][e] * g.xv[d].dx[I[d]];
 }
 
-__global__ void k_scale(const double* __restrict__ V, float* __restrict__ Vs, int64_t n,
-                        double dt, const double* __restrict__ h) {
-    const double s = dt / h[0];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        Vs[i] = (float)(V[i] * s);
+// ------------------------------------------------------------ advection
+// One workgroup per tile of cells (2D 64 x 4, 3D 16 x 4 x 4), one thread per
+// cell.  The tile's window of every velocity component, nodes [t0 - 1,
+// t0 + T + 3) per dimension, is staged in LDS already scaled to cells per
+// step, float(V_c * dt / h_c).  That window covers every stage point within one
+// cell of the tile (vertex axis: nodes J, J+1; centre axis: J+up, J+up+1);
+// points further out read the same values from global memory.  Each thread
+// walks its cell's slots in chunks of 8 planes, in lockstep with the
+// neighbouring cells of its wavefront.
+template <int D>
+__host__ __device__ constexpr int tile_n(int d) {
+    return D == 2 ? (d == 0 ? 64 : 4) : (d == 0 ? 16 : 4);
+}
+template <int D>
+__host__ __device__ constexpr int win_n(int d) {
+    return tile_n<D>(d) + 4;
+}
+template <int D>
+__host__ __device__ constexpr int win_size() {
+    return D == 2 ? win_n<2>(0) * win_n<2>(1) : win_n<3>(0) * win_n<3>(1) * win_n<3>(2);
 }
 
-// ------------------------------------------------------------ advection
-// velocity of component c at cell I + offset u (u in cell units, any value)
+// FP64 input velocity (component grids n[c][d]) and the vertex spacings
 template <int D>
-__device__ inline float interp_split(const SVel<D>& V, const SGrid<D>& g, int c, const int (&I)[D],
-                                     const float (&u)[D]) {
+struct SVelIn {
+    const double* v[D];
+    int n[D][D];
+};
+
+// velocity of component c at cell I + offset u (u in cell units, any value),
+// from the LDS window sw (origin t0 - 1) when it holds the four/eight corners
+template <int D>
+__device__ inline float interp_tile(const float* __restrict__ sw, const SVelIn<D>& V,
+                                    const SGrid<D>& g, int c, double sc, const int (&t0)[D],
+                                    const int (&I)[D], const float (&u)[D]) {
     int k[D];
     float t[D];
+    bool inw = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         if (!(fabsf(u[d]) < 1.0e6f)) return __builtin_inff();
@@ -118,19 +134,35 @@ __device__ inline float interp_split(const SVel<D>& V, const SGrid<D>& g, int c,
             if (k[d] < 0 || k[d] > g.ni[d]) return __builtin_inff();
             t[d] = up ? f - 0.5f : f + 0.5f;
         }
+        const int kl = k[d] - (t0[d] - 1);
+        inw = inw && kl >= 0 && kl + 1 < win_n<D>(d);
     }
-    const float* __restrict__ F = V.v[c];
-    const int n0 = V.n[c][0];
-    if constexpr (D == 2) {
-        const int b = k[0] + n0 * k[1];
-        return lerp1f(t[1], lerp1f(t[0], F[b], F[b + 1]), lerp1f(t[0], F[b + n0], F[b + n0 + 1]));
+    float q[1 << D];
+    if (inw) {
+        constexpr int W0 = win_n<D>(0), W1 = win_n<D>(1);
+        int b = (k[0] - t0[0] + 1) + W0 * (k[1] - t0[1] + 1);
+        if constexpr (D == 3) b += W0 * W1 * (k[2] - t0[2] + 1);
+#pragma unroll
+        for (int r = 0; r < (1 << D); ++r) {
+            const int o = (r & 1) + ((r >> 1) & 1) * W0 + (D == 3 ? ((r >> 2) & 1) * W0 * W1 : 0);
+            q[r] = sw[b + o];
+        }
     } else {
-        const int n1 = V.n[c][1], s2 = n0 * n1;
-        const int b = k[0] + n0 * (k[1] + n1 * k[2]);
-        const float lo =
-            lerp1f(t[1], lerp1f(t[0], F[b], F[b + 1]), lerp1f(t[0], F[b + n0], F[b + n0 + 1]));
-        const float hi = lerp1f(t[1], lerp1f(t[0], F[b + s2], F[b + s2 + 1]),
-                                lerp1f(t[0], F[b + s2 + n0], F[b + s2 + n0 + 1]));
+        const double* __restrict__ F = V.v[c];
+        const int64_t n0 = V.n[c][0], n1 = V.n[c][1];
+        int64_t b = k[0] + n0 * (int64_t)k[1];
+        if constexpr (D == 3) b += n0 * n1 * (int64_t)k[2];
+#pragma unroll
+        for (int r = 0; r < (1 << D); ++r) {
+            const int64_t o = (r & 1) + ((r >> 1) & 1) * n0 + (D == 3 ? ((r >> 2) & 1) * n0 * n1 : 0);
+            q[r] = (float)(F[b + o] * sc);
+        }
+    }
+    const float lo = lerp1f(t[1], lerp1f(t[0], q[0], q[1]), lerp1f(t[0], q[2], q[3]));
+    if constexpr (D == 2) {
+        return lo;
+    } else {
+        const float hi = lerp1f(t[1], lerp1f(t[0], q[4], q[5]), lerp1f(t[0], q[6], q[7]));
         return lerp1f(t[2], lo, hi);
     }
 }
@@ -142,22 +174,13 @@ struct RKf {
 };
 
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) k_adv_split(Grid<D> gg, SGrid<D> g, SVel<D> V,
-                                                      FPtr3 off, const uint8_t* __restrict__ index,
-                                                      RKf P) {
-    int64_t c, I64[D];
-    if (!row_cell<D>(gg, c, I64)) return;
-    const int64_t e = c + g.C * (int64_t)blockIdx.y;
-    if (!index[e]) return;
-    int I[D];
-    float u0[D], v0[D], out[D];
+__device__ inline void advect_split_one(const float (*sV)[win_size<D>()], const SVelIn<D>& V,
+                                        const SGrid<D>& g, const double (&sc)[D],
+                                        const int (&t0)[D], const int (&I)[D], const RKf& P,
+                                        const float (&u0)[D], float (&out)[D]) {
+    float v0[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        I[d] = (int)I64[d];
-        u0[d] = off.p[d][e];
-    }
-#pragma unroll
-    for (int cc = 0; cc < D; ++cc) v0[cc] = interp_split<D>(V, g, cc, I, u0);
+    for (int cc = 0; cc < D; ++cc) v0[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, u0);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = u0[d] + v0[d];
@@ -166,7 +189,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(Grid<D> gg, SGrid<D> g, SV
 #pragma unroll
         for (int d = 0; d < D; ++d) u1[d] = fmaf(P.alpha, v0[d], u0[d]);
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) v1[cc] = interp_split<D>(V, g, cc, I, u1);
+        for (int cc = 0; cc < D; ++cc) v1[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, u1);
         if (P.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = u0[d] + v1[d];
@@ -179,21 +202,103 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(Grid<D> gg, SGrid<D> g, SV
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + 0.5f * v0[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k2[cc] = interp_split<D>(V, g, cc, I, q);
+        for (int cc = 0; cc < D; ++cc) k2[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, q);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + 0.5f * k2[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k3[cc] = interp_split<D>(V, g, cc, I, q);
+        for (int cc = 0; cc < D; ++cc) k3[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, q);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + k3[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k4[cc] = interp_split<D>(V, g, cc, I, q);
+        for (int cc = 0; cc < D; ++cc) k4[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, q);
 #pragma unroll
         for (int d = 0; d < D; ++d)
             out[d] = u0[d] + (((v0[d] + 2.0f * k2[d]) + 2.0f * k3[d]) + k4[d]) / 6.0f;
     }
+}
+
+template <int D, int M>
+__global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, double dt,
+                                                      FPtr3 off, const uint8_t* __restrict__ index,
+                                                      RKf P) {
+    __shared__ float sV[D][win_size<D>()];
+    int t0[D];
+    {
+        int b = blockIdx.x;
 #pragma unroll
-    for (int d = 0; d < D; ++d) off.p[d][e] = out[d];
+        for (int d = 0; d < D; ++d) {
+            const int nt = (g.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
+            t0[d] = (b % nt) * tile_n<D>(d);
+            b /= nt;
+        }
+    }
+    double sc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) sc[c] = dt / g.h[c][0];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const double* __restrict__ F = V.v[c];
+        for (int i = tid; i < win_size<D>(); i += kBlock) {
+            int r = i;
+            int64_t node = 0, stride = 1;
+            bool ok = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int k = t0[d] - 1 + r % win_n<D>(d);
+                r /= win_n<D>(d);
+                ok = ok && k >= 0 && k < V.n[c][d];
+                node += k * stride;
+                stride *= V.n[c][d];
+            }
+            sV[c][i] = ok ? (float)(F[node] * sc[c]) : 0.0f;
+        }
+    }
+    __syncthreads();
+    int I[D];
+    bool in = true;
+    {
+        int r = tid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            I[d] = t0[d] + r % tile_n<D>(d);
+            r /= tile_n<D>(d);
+            in = in && I[d] < g.ni[d];
+        }
+    }
+    if (!in) return;
+    int64_t c = I[D - 1];
+#pragma unroll
+    for (int d = D - 2; d >= 0; --d) c = c * g.ni[d] + I[d];
+    const int64_t C = g.C;
+    const uint64_t act = active_slots(index, c, C, g.K);
+    constexpr int B = 8;
+    for (int k0 = 0; k0 < g.K; k0 += B) {
+        const uint32_t ab = (uint32_t)(act >> k0) & 0xFFu;
+        if (!ab) continue;
+        float ub[B][D];
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+#pragma unroll
+            for (int d = 0; d < D; ++d) ub[j][d] = ((ab >> j) & 1) ? off.p[d][c + C * (k0 + j)] : 0.0f;
+        // one particle at a time (the chunk's loads are already in flight)
+        uint32_t todo = ab;
+        while (todo) {
+            const int j = __builtin_ctz(todo);
+            todo &= todo - 1;
+            float u0[D], out[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                float v = ub[0][d];
+#pragma unroll
+                for (int q = 1; q < B; ++q) v = j == q ? ub[q][d] : v;
+                u0[d] = v;
+            }
+            advect_split_one<D, M>(sV, V, g, sc, t0, I, P, u0, out);
+#pragma unroll
+            for (int d = 0; d < D; ++d) off.p[d][c + C * (k0 + j)] = out[d];
+        }
+    }
 }
 
 // ------------------------------------------------------------ particle2grid
@@ -202,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(Grid<D> gg, SGrid<D> g, SV
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_p2g_split(Grid<D> gg, SGrid<D> g, double* __restrict__ F,
                                                       const double* __restrict__ Fp, FPtr3 off,
-                                                      const uint8_t* __restrict__ index) {
+                                                      const uint64_t* __restrict__ occw) {
     int64_t nn[D];
     int64_t NN = 1;
 #pragma unroll
@@ -225,49 +330,83 @@ __global__ void __launch_bounds__(kBlock) k_p2g_split(Grid<D> gg, SGrid<D> g, do
 #pragma unroll
     for (int d = 0; d < D; ++d) hf[d] = (float)g.h[d][0];
     const int64_t C = g.C;
-    double W = 0.0, WF = 0.0;
     constexpr int NC = 1 << D;
-#pragma unroll 1
+    int64_t cell[NC];
+    uint64_t bits[NC];
+#pragma unroll
     for (int q = 0; q < NC; ++q) {
         int64_t I[D];
-        float a[D];  // node - cell corner, in cells: 0 or 1
         bool ok = true;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int o = ((q >> d) & 1) ? 0 : 1;
-            I[d] = N[d] - o;
-            a[d] = (float)o;
+            I[d] = N[d] + (((q >> d) & 1) ? 0 : -1);
             ok = ok && (I[d] >= 0 && I[d] < g.ni[d]);
         }
-        if (!ok) continue;
-        const int64_t c = lin<D>(gg.ni, I);
-        const uint64_t bq = active_slots(index, c, C, g.K);
-        constexpr int B = 16;
-        for (int k0 = 0; k0 < g.K; k0 += B) {
-            float ub[B][D];
-            double fb[B];
+        cell[q] = ok ? lin<D>(gg.ni, I) : 0;
+        bits[q] = ok ? occw[cell[q]] : 0ull;
+    }
+    double W = 0.0, WF = 0.0;
+    constexpr int B = D == 2 ? 8 : 4;
+    const int nk = (g.K + B - 1) / B;
+    const int total = NC * nk;
+    auto chunk = [&](int t, int64_t& c, uint32_t& bq, int& k0, int& q) __attribute__((always_inline)) {
+        q = t / nk;
+        k0 = (t - q * nk) * B;
+        c = cell[0];
+        uint64_t bw = bits[0];
 #pragma unroll
-            for (int j = 0; j < B; ++j) {
-                const bool on = (bq >> (k0 + j)) & 1;
-                const int64_t e = c + C * (k0 + j);
-#pragma unroll
-                for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][e] : 0.0f;
-                fb[j] = on ? Fp[e] : 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                if (!((bq >> (k0 + j)) & 1)) continue;
-                float s = 0.0f;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const float r = (a[d] - ub[j][d]) * hf[d];
-                    s = fmaf(r, r, s);
-                }
-                const double w = (double)(1.0f / s);
-                W += w;
-                WF = fma(w, fb[j], WF);
-            }
+        for (int u = 1; u < NC; ++u) {
+            c = q == u ? cell[u] : c;
+            bw = q == u ? bits[u] : bw;
         }
+        bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
+    };
+    auto load = [&](int t, float (&ub)[B][D], double (&fb)[B]) __attribute__((always_inline)) {
+        int64_t c;
+        uint32_t bq;
+        int k0, q;
+        chunk(t, c, bq, k0, q);
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const bool on = (bq >> j) & 1;
+            const int64_t e = c + C * (k0 + j);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][e] : 0.0f;
+            fb[j] = on ? Fp[e] : 0.0;
+        }
+    };
+    auto accumulate = [&](int t, const float (&ub)[B][D], const double (&fb)[B])
+                          __attribute__((always_inline)) {
+        int64_t c;
+        uint32_t bq;
+        int k0, q;
+        chunk(t, c, bq, k0, q);
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (!((bq >> j) & 1)) continue;
+            float s = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                // node minus particle in cells: the node is corner 1 - bit of the cell
+                const float a = (((q >> d) & 1) ? 0.0f : 1.0f) - ub[j][d];
+                const float r = a * hf[d];
+                s = fmaf(r, r, s);
+            }
+            const double w = (double)(1.0f / s);
+            W += w;
+            WF = fma(w, fb[j], WF);
+        }
+    };
+    float u0[B][D], u1[B][D];
+    double f0[B], f1[B];
+    load(0, u0, f0);
+#pragma unroll 1
+    for (int t = 0; t < total; t += 2) {
+        if (t + 1 < total) load(t + 1, u1, f1);
+        accumulate(t, u0, f0);
+        if (t + 1 >= total) break;
+        if (t + 2 < total) load(t + 2, u0, f0);
+        accumulate(t + 1, u1, f1);
     }
     F[node] = WF / W;
 }
@@ -308,38 +447,37 @@ __global__ void __launch_bounds__(kBlock) k_g2p_split(Grid<D> g, double* __restr
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_inj_scan_split(Grid<D> g, FPtr3 off,
                                                            const uint8_t* __restrict__ index,
-                                                           int min_q, int32_t* __restrict__ lists,
-                                                           int32_t* __restrict__ counts,
-                                                           int64_t cap) {
+                                                           int min_q, InjectLists L) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= g.C) return;
+    const bool valid = c < g.C;  // all lanes stay for the wavefront append
     int64_t I[D];
-    unlin<D>(g.ni, c, I);
-    const uint64_t act = active_slots(index, c, g.C, g.K);
+    unlin<D>(g.ni, valid ? c : 0, I);
     int num = 0;
-    constexpr int B = 16;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
-        float ub[B][D];
+    if (valid) {
+        const uint64_t act = active_slots(index, c, g.C, g.K);
+        constexpr int B = 16;
+        for (int k0 = 0; k0 < g.K; k0 += B) {
+            float ub[B][D];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            const bool on = (act >> (k0 + j)) & 1;
+            for (int j = 0; j < B; ++j) {
+                const bool on = (act >> (k0 + j)) & 1;
 #pragma unroll
-            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + g.C * (k0 + j)] : 0.0f;
-        }
+                for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + g.C * (k0 + j)] : 0.0f;
+            }
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            bool inside = (act >> (k0 + j)) & 1;
+            for (int j = 0; j < B; ++j) {
+                bool inside = (act >> (k0 + j)) & 1;
 #pragma unroll
-            for (int d = 0; d < D; ++d) inside = inside & (0.0f < ub[j][d] && ub[j][d] < 0.5f);
-            num += inside ? 1 : 0;
+                for (int d = 0; d < D; ++d) inside = inside & (0.0f < ub[j][d] && ub[j][d] < 0.5f);
+                num += inside ? 1 : 0;
+            }
         }
     }
-    if (num >= min_q) return;
     int col = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) col = col * 2 + (int)(I[d] & 1);
-    const int32_t slot = atomicAdd(&counts[col], 1);
-    lists[col * cap + slot] = (int32_t)c;
+    wave_segment<D == 2 ? 4 : 8>(valid && num < min_q, col, (int32_t)c, c >> 6, L.nwave, L.seg,
+                                 L.cnt);
 }
 
 template <int D>
@@ -351,8 +489,13 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave_split(Grid<D> g, SGrid<D> s
                                                            const int32_t* __restrict__ count,
                                                            jp_stats* st) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6);
-    if (w >= *count) return;
+    // persistent wavefronts over the colour's list (every wavefront exits once
+    // the list is exhausted)
+    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t total = *count;
+    int64_t n_inj = 0, n_orph = 0;
+    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
+         w += nwv) {
     const int64_t c = list[w];
     const int64_t C = g.C;
     const int K = g.K;
@@ -368,7 +511,6 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave_split(Grid<D> g, SGrid<D> s
     float u[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) u[d] = act ? off.p[d][e] : 0.0f;
-    int64_t n_inj = 0, n_orph = 0;
     for (int q = 0; q < NQ; ++q) {
         float q0[D];
 #pragma unroll
@@ -401,42 +543,52 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave_split(Grid<D> g, SGrid<D> s
             }
             double best = __builtin_inf();
             int border = 0x7fffffff;
-            for (int nb = 0; nb < NB; ++nb) {
-                int64_t J[D];
-                int o[D];
-                bool ok = own;
-                int r = nb;
+            constexpr int NBB = 9;  // neighbour cells per batch, one round of loads each
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    o[d] = (r % 3) - 1;
-                    J[d] = I[d] + o[d];
-                    r /= 3;
-                    ok = ok && J[d] >= 0 && J[d] < g.ni[d];
+            for (int nb0 = 0; nb0 < NB; nb0 += NBB) {
+                bool a[NBB];
+                int64_t e2[NBB];
+                int o[NBB][D];
+#pragma unroll
+                for (int v = 0; v < NBB; ++v) {
+                    int64_t J[D];
+                    bool ok = own;
+                    int r = nb0 + v;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        o[v][d] = (r % 3) - 1;
+                        J[d] = I[d] + o[v][d];
+                        r /= 3;
+                        ok = ok && J[d] >= 0 && J[d] < g.ni[d];
+                    }
+                    e2[v] = ok ? lin<D>(g.ni, J) + C * lane : 0;
+                    a[v] = ok;
                 }
-                if (!ok) continue;
-                bool a;
-                float uq[D];
-                if (nb == CENTER) {
-                    a = act && lane != ip;
+                uint8_t m[NBB];
+                float uq[NBB][D];
 #pragma unroll
-                    for (int d = 0; d < D; ++d) uq[d] = u[d];
-                } else {
-                    const int64_t e2 = lin<D>(g.ni, J) + C * lane;
-                    a = index[e2] != 0;
+                for (int v = 0; v < NBB; ++v) {
+                    const bool other = a[v] && nb0 + v != CENTER;
+                    m[v] = other ? index[e2[v]] : (uint8_t)0;
 #pragma unroll
-                    for (int d = 0; d < D; ++d) uq[d] = a ? off.p[d][e2] : 0.0f;
+                    for (int d = 0; d < D; ++d) uq[v][d] = other ? off.p[d][e2[v]] : u[d];
                 }
-                if (!a) continue;
-                double s = 0.0;
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const double r2 = ((double)o[d] + (double)uq[d] - (double)un[d]) * sg.h[d][0];
-                    s = s + r2 * r2;
-                }
-                const double dd = sqrt(s);
-                if (dd < best) {
-                    best = dd;
-                    border = nb * 64 + lane;
+                for (int v = 0; v < NBB; ++v) {
+                    const bool av = (nb0 + v == CENTER) ? (act && lane != ip) : (m[v] != 0);
+                    if (!av) continue;
+                    double s2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const double r2 =
+                            ((double)o[v][d] + (double)uq[v][d] - (double)un[d]) * sg.h[d][0];
+                        s2 = s2 + r2 * r2;
+                    }
+                    const double dd = sqrt(s2);
+                    if (dd < best) {
+                        best = dd;
+                        border = (nb0 + v) * 64 + lane;
+                    }
                 }
             }
 #pragma unroll
@@ -466,6 +618,118 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave_split(Grid<D> g, SGrid<D> s
             if (num >= min_q) break;
         }
     }
+    }  // work list
+    if (lane == 0) {
+        stat_add(st, ST_INJECTED, n_inj);
+        stat_add(st, ST_ORPHAN, n_orph);
+    }
+}
+
+// 2D with at most one field: the 3x3 neighbourhood (mask, offsets, field) is
+// loaded once per listed cell into registers; nearest-neighbour copies come
+// from registers through a shuffle.
+__global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> sg, FPtr3 off,
+                                                            uint8_t* index, int nargs,
+                                                            double* __restrict__ arg, int min_q,
+                                                            uint64_t seed, uint64_t counter,
+                                                            const int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ count,
+                                                            jp_stats* st) {
+    constexpr int NB = 9, CENTER = 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t total = *count;
+    const int64_t C = g.C;
+    const int K = g.K;
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    const bool own = lane < K;
+    const double h0 = sg.h[0][0], h1 = sg.h[1][0];
+    int64_t n_inj = 0, n_orph = 0;
+    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
+         w += nwv) {
+        const int64_t c = __builtin_amdgcn_readfirstlane(list[w]);
+        int64_t I[2];
+        unlin<2>(g.ni, c, I);
+        uint8_t m[NB];
+        float uq[NB][2];
+        double a[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
+            const bool ok = own && J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            const int64_t e2 = J0 + g.ni[0] * J1 + C * lane;
+            m[nb] = ok ? index[e2] : (uint8_t)0;
+            uq[nb][0] = ok ? off.p[0][e2] : 0.0f;
+            uq[nb][1] = ok ? off.p[1][e2] : 0.0f;
+            a[nb] = (ok && nargs) ? arg[e2] : 0.0;
+        }
+        bool act = m[CENTER] != 0;
+        for (int qd = 0; qd < 4; ++qd) {
+            const float q0 = 0.5f * (float)(qd & 1), q1 = 0.5f * (float)((qd >> 1) & 1);
+            const bool inside = act && (q0 < uq[CENTER][0] && uq[CENTER][0] < q0 + 0.5f) &&
+                                (q1 < uq[CENTER][1] && uq[CENTER][1] < q1 + 0.5f);
+            int num = __popcll(__ballot(inside));
+            if (num >= min_q) break;
+            uint64_t freeS = ~__ballot(act) & full;
+            while (freeS) {
+                const int ip = __builtin_ctzll(freeS);
+                freeS &= freeS - 1;
+                ++num;
+                float un[2];
+                un[0] = (float)((double)q0 + 0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 0, 2, counter), 0.05));
+                un[1] = (float)((double)q1 + 0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 1, 2, counter), 0.05));
+                const int64_t enew = c + C * ip;
+                if (lane == ip) {
+                    act = true;
+                    uq[CENTER][0] = un[0];
+                    uq[CENTER][1] = un[1];
+                    off.p[0][enew] = un[0];
+                    off.p[1][enew] = un[1];
+                    index[enew] = 1;
+                }
+                double best = __builtin_inf();
+                int border = 0x7fffffff;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const bool av = nb == CENTER ? (act && lane != ip) : (m[nb] != 0);
+                    if (!av) continue;
+                    const double r0 =
+                        ((double)(nb % 3 - 1) + (double)uq[nb][0] - (double)un[0]) * h0;
+                    const double r1 =
+                        ((double)(nb / 3 - 1) + (double)uq[nb][1] - (double)un[1]) * h1;
+                    const double dd = sqrt(r0 * r0 + r1 * r1);
+                    if (dd < best) {
+                        best = dd;
+                        border = nb * 64 + lane;
+                    }
+                }
+#pragma unroll
+                for (int s = 32; s >= 1; s >>= 1) {
+                    const double ob = __shfl_xor(best, s, 64);
+                    const int oo = __shfl_xor(border, s, 64);
+                    if (ob < best || (ob == best && oo < border)) {
+                        best = ob;
+                        border = oo;
+                    }
+                }
+                if (border == 0x7fffffff) {
+                    ++n_orph;
+                } else if (nargs) {
+                    const int nbw = border >> 6;
+                    double mine = a[0];
+#pragma unroll
+                    for (int nb = 1; nb < NB; ++nb) mine = nb == nbw ? a[nb] : mine;
+                    const double v = __shfl(mine, border & 63, 64);
+                    if (lane == ip) {
+                        a[CENTER] = v;
+                        arg[enew] = v;
+                    }
+                }
+                ++n_inj;
+                if (num >= min_q) break;
+            }
+        }
+    }  // work list
     if (lane == 0) {
         stat_add(st, ST_INJECTED, n_inj);
         stat_add(st, ST_ORPHAN, n_orph);
@@ -494,16 +758,12 @@ template <int D>
 int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t* index,
             const double* const* V, const jp_velgrid* vg, int method, double alpha, double dt) {
     Grid<D> G = to_grid<D>(gc);
-    int rc;
     SGrid<D> sg = sgrid<D>(G);
-    SVel<D> sv;
-    size_t tot = 0, offs[3];
-    int64_t cnt[3];
+    SVelIn<D> sv;
     for (int c = 0; c < D; ++c) {
-        int64_t n = 1;
+        sv.v[c] = V[c];
         for (int d = 0; d < D; ++d) {
             sv.n[c][d] = (int)vg->ax[c][d].n;
-            n *= vg->ax[c][d].n;
             // standard staggering: the vertex axis along c, centres + ghosts elsewhere
             const int64_t want = d == c ? G.ni[d] + 1 : G.ni[d] + 2;
             if (vg->ax[c][d].n != want)
@@ -511,19 +771,6 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
                             "split advection needs the standard staggered grids%s (component %lld)",
                             "", c);
         }
-        offs[c] = tot;
-        cnt[c] = n;
-        tot += (size_t)n * 4;
-        tot = (tot + 255) & ~(size_t)255;
-    }
-    if ((rc = ensure_scratch(ctx, tot))) return rc;
-    for (int c = 0; c < D; ++c) {
-        float* vs = reinterpret_cast<float*>(static_cast<char*>(ctx->scratch) + offs[c]);
-        int64_t b = blocks_for(cnt[c]);
-        k_scale<<<b > 65536 ? 65536 : b, kBlock, 0, ctx->stream>>>(V[c], vs, cnt[c], dt,
-                                                                    gc->xv[c].dx);
-        JP_LAUNCHED(ctx);
-        sv.v[c] = vs;
     }
     RKf P;
     P.alpha = (float)alpha;
@@ -531,14 +778,15 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     const double ia = 1.0 / alpha;
     P.w0 = (float)std::fma(-0.5, ia, 1.0);
     P.w1 = (float)(0.5 * ia);
-    dim3 grid((unsigned)row_blocks<D>(G), (unsigned)G.K);
+    int64_t tiles = 1;
+    for (int d = 0; d < D; ++d) tiles *= (G.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
     FPtr3 fp = fptr3<D>(off);
     if (method == JP_EULER)
-        k_adv_split<D, JP_EULER><<<grid, kBlock, 0, ctx->stream>>>(G, sg, sv, fp, index, P);
+        k_adv_split<D, JP_EULER><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
     else if (method == JP_RK2)
-        k_adv_split<D, JP_RK2><<<grid, kBlock, 0, ctx->stream>>>(G, sg, sv, fp, index, P);
+        k_adv_split<D, JP_RK2><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
     else
-        k_adv_split<D, JP_RK4><<<grid, kBlock, 0, ctx->stream>>>(G, sg, sv, fp, index, P);
+        k_adv_split<D, JP_RK4><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
     JP_LAUNCHED(ctx);
     return finish(ctx);
 }
@@ -550,7 +798,11 @@ int run_p2g(jp_ctx* ctx, const jp_cellgrid* gc, double* F, const double* Fp, flo
     SGrid<D> sg = sgrid<D>(G);
     int64_t NN = 1;
     for (int d = 0; d < D; ++d) NN *= G.ni[d] + 1;
-    k_p2g_split<D><<<blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, F, Fp, fptr3<D>(off), index);
+    int rc;
+    if ((rc = ensure_scratch(ctx, (size_t)G.C * 8))) return rc;
+    uint64_t* occ = static_cast<uint64_t*>(ctx->scratch);
+    if ((rc = occ_words(ctx, index, G.C, G.K, occ))) return rc;
+    k_p2g_split<D><<<blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, F, Fp, fptr3<D>(off), occ);
     JP_LAUNCHED(ctx);
     return finish(ctx);
 }
@@ -577,22 +829,28 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     const int min_q = (min_xcell + NQ - 1) / NQ;
     int64_t cap = 1;
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
-    const size_t counts_bytes = 256;
-    if ((rc = ensure_scratch(ctx, counts_bytes + (size_t)NQ * cap * 4))) return rc;
-    int32_t* counts = static_cast<int32_t*>(ctx->scratch);
-    int32_t* lists = reinterpret_cast<int32_t*>(static_cast<char*>(ctx->scratch) + counts_bytes);
-    if ((rc = zero_bytes(ctx, counts, counts_bytes))) return rc;
+    InjectLists L;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, L))) return rc;
     FPtr3 fp = fptr3<D>(off);
-    k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, lists,
-                                                                     counts, cap);
+    k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, L);
     JP_LAUNCHED(ctx);
+    if ((rc = join_segments(ctx, NQ, L))) return rc;
     ArgPtrs ap;
     for (int i = 0; i < JP_MAX_ARGS; ++i) ap.p[i] = i < nargs ? args[i] : nullptr;
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     for (int ph = 0; ph < NQ; ++ph) {
-        k_inj_wave_split<D><<<blocks_for(cap, kBlock / 64), kBlock, 0, ctx->stream>>>(
-            G, sg, fp, index, nargs, ap, min_q, seed, counter, lists + (int64_t)ph * cap,
-            counts + ph, st);
+        if constexpr (D == 2) {
+            if (nargs <= 1) {
+                k_inj_wave2_split<<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+                    G, sg, fp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
+                    L.list + (int64_t)ph * cap, L.total + ph, st);
+                JP_LAUNCHED(ctx);
+                continue;
+            }
+        }
+        k_inj_wave_split<D><<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+            G, sg, fp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
+            L.total + ph, st);
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;
