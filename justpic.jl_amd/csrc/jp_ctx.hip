@@ -95,20 +95,24 @@ int occ_words(jp_ctx* ctx, const uint8_t* index, int64_t C, int K, uint64_t* occ
     return JP_OK;
 }
 
-int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, InjectLists& L) {
-    // scratch: totals | counts | segments | lists
+int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, InjectLists& L) {
+    // scratch: totals | counts | segments | lists | occupancy words | boxes
     L.nwave = (C + 63) / 64;
     const size_t a = 256;
     const size_t b = a + (((size_t)nq * L.nwave * 4 + 255) & ~(size_t)255);
     const size_t c = b + (size_t)nq * L.nwave * 64 * 4;
-    const size_t d = c + (size_t)nq * cap * 4;
-    int rc = ensure_scratch(ctx, d);
+    const size_t d = c + (((size_t)nq * cap * 4 + 255) & ~(size_t)255);
+    const size_t e = d + (((size_t)C * 8 + 255) & ~(size_t)255);
+    const size_t f = e + box_bytes;
+    int rc = ensure_scratch(ctx, f);
     if (rc) return rc;
     char* base = static_cast<char*>(ctx->scratch);
     L.total = reinterpret_cast<int32_t*>(base);
     L.cnt = reinterpret_cast<int32_t*>(base + a);
     L.seg = reinterpret_cast<int32_t*>(base + b);
     L.list = reinterpret_cast<int32_t*>(base + c);
+    L.occ = reinterpret_cast<uint64_t*>(base + d);
+    L.box = box_bytes ? static_cast<void*>(base + e) : nullptr;
     L.cap = cap;
     return JP_OK;
 }

@@ -60,8 +60,10 @@ struct InjectLists {
     int32_t* list;
     int64_t nwave;
     int64_t cap;
+    uint64_t* occ;  // occupancy word of every cell
+    void* box;      // 2*D planes of C: per-cell min, max of each coordinate (2D)
 };
-int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, InjectLists& L);
+int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, InjectLists& L);
 int join_segments(jp_ctx* ctx, int nq, const InjectLists& L);
 // persistent wavefronts for a work list of at most n entries
 inline int64_t inject_blocks(int64_t n) {

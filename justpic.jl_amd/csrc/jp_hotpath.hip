@@ -235,6 +235,12 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
             q1[d] = q0[d] + hq[d];
         }
         const uint64_t act = active_slots(index, c, g.C, g.K);
+        double bl[D], bh[D];  // box of the cell's particles (NaN coordinates ignored)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            bl[d] = __builtin_inf();
+            bh[d] = -__builtin_inf();
+        }
         constexpr int B = 16;  // slots per chunk; lanes share the slot plane (coalesced)
         for (int k0 = 0; k0 < g.K; k0 += B) {
             double pb[B][D];
@@ -246,10 +252,26 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
             }
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                bool inside = (act >> (k0 + j)) & 1;
+                const bool on = (act >> (k0 + j)) & 1;
+                bool inside = on;
 #pragma unroll
-                for (int d = 0; d < D; ++d) inside = inside & (q0[d] < pb[j][d] && pb[j][d] < q1[d]);
+                for (int d = 0; d < D; ++d) {
+                    inside = inside & (q0[d] < pb[j][d] && pb[j][d] < q1[d]);
+                    if (D == 2 && on) {
+                        bl[d] = fmin(bl[d], pb[j][d]);
+                        bh[d] = fmax(bh[d], pb[j][d]);
+                    }
+                }
                 num += inside ? 1 : 0;
+            }
+        }
+        L.occ[c] = act;
+        if (D == 2) {
+            double* __restrict__ bx = static_cast<double*>(L.box);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                bx[(2 * d) * g.C + c] = bl[d];
+                bx[(2 * d + 1) * g.C + c] = bh[d];
             }
         }
     }
@@ -432,16 +454,24 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
     }
 }
 
-// The same injection for 2D with at most one field, the neighbourhood held in
-// registers: each lane loads its slot of the 3x3 cells (mask, coordinates
-// and the field) once per listed cell; nearest-neighbour copies then come
-// from registers through a shuffle, so an injection issues only stores.
+// The same injection for 2D with at most one field, loading only what the
+// nearest-neighbour search can use.  The scan wrote every cell's occupancy
+// word and the box of its particles; a wavefront loads its cell's particles
+// (active slots only), and a neighbour cell's particles only when the
+// distance from the new particle to that cell's box does not exceed the best
+// distance found so far (the box bound is computed with the same floating
+// point operations as the distances, so the pruned search returns exactly the
+// reference's first-occurrence minimum).  Loaded neighbours stay in
+// registers for the cell's later injections (neighbours do not change during
+// a colour phase), and copies come from registers or one uniform load.
 __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords, uint8_t* index,
                                                          int nargs, double* __restrict__ arg,
                                                          int min_q, uint64_t seed,
                                                          uint64_t counter,
                                                          const int32_t* __restrict__ list,
                                                          const int32_t* __restrict__ count,
+                                                         uint64_t* __restrict__ occw,
+                                                         double* __restrict__ box,
                                                          jp_stats* st) {
     constexpr int NB = 9, CENTER = 4;
     const int lane = threadIdx.x & 63;
@@ -450,7 +480,6 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
     const int64_t C = g.C;
     const int K = g.K;
     const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
-    const bool own = lane < K;
     int64_t n_inj = 0, n_orph = 0;
     for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
          w += nwv) {
@@ -463,20 +492,30 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
             xv[d] = g.xv[d].x[I[d]];
             dq[d] = g.xv[d].dx[I[d]] / 2.0;
         }
-        // lane's slot of the 3x3 neighbourhood, one round of loads
-        uint8_t m[NB];
-        double q[NB][2], a[NB];
+        int64_t cn[NB];
+        uint64_t o9[NB];
+        double bl[NB][2], bh[NB][2];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
-            const bool ok = own && J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
-            const int64_t e2 = J0 + g.ni[0] * J1 + C * lane;
-            m[nb] = ok ? index[e2] : (uint8_t)0;
-            q[nb][0] = ok ? coords.p[0][e2] : 0.0;
-            q[nb][1] = ok ? coords.p[1][e2] : 0.0;
-            a[nb] = (ok && nargs) ? arg[e2] : 0.0;
+            const bool ok = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            cn[nb] = ok ? J0 + g.ni[0] * J1 : 0;
+            o9[nb] = ok ? occw[cn[nb]] : 0ull;
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                bl[nb][d] = ok ? box[(2 * d) * C + cn[nb]] : 0.0;
+                bh[nb][d] = ok ? box[(2 * d + 1) * C + cn[nb]] : 0.0;
+            }
         }
-        bool act = m[CENTER] != 0;
+        const int64_t e = c + C * lane;
+        bool act = (o9[CENTER] >> lane) & 1;
+        double q[NB][2];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) q[nb][0] = q[nb][1] = 0.0;
+        q[CENTER][0] = act ? coords.p[0][e] : 0.0;
+        q[CENTER][1] = act ? coords.p[1][e] : 0.0;
+        double a_c = (act && nargs) ? arg[e] : 0.0;
+        uint32_t loaded = 1u << CENTER;
         for (int qd = 0; qd < 4; ++qd) {
             double vert[2];
 #pragma unroll
@@ -499,6 +538,10 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                     pn[d] = vert[d] + dq[d] * fma(0.95, r, 0.05);
                 }
                 const int64_t enew = c + C * ip;
+                bl[CENTER][0] = fmin(bl[CENTER][0], pn[0]);
+                bh[CENTER][0] = fmax(bh[CENTER][0], pn[0]);
+                bl[CENTER][1] = fmin(bl[CENTER][1], pn[1]);
+                bh[CENTER][1] = fmax(bh[CENTER][1], pn[1]);
                 if (lane == ip) {
                     act = true;
                     q[CENTER][0] = pn[0];
@@ -507,46 +550,90 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                     coords.p[1][enew] = pn[1];
                     index[enew] = 1;
                 }
-                // index_min_distance: first occurrence in (cell, slot) order
-                double best = __builtin_inf();
-                int border = 0x7fffffff;
+                // index_min_distance over the loaded cells: first occurrence
+                // of the minimum in (cell, slot) order
+                double best;
+                int border;
+                auto search = [&]() __attribute__((always_inline)) {
+                    best = __builtin_inf();
+                    border = 0x7fffffff;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if (!((loaded >> nb) & 1)) continue;
+                        const bool av = nb == CENTER ? (act && lane != ip) : ((o9[nb] >> lane) & 1);
+                        if (!av) continue;
+                        if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
+                        const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
+                        const double dd = sqrt(du * du + dv * dv);
+                        if (dd < best) {
+                            best = dd;
+                            border = nb * 64 + lane;
+                        }
+                    }
+#pragma unroll
+                    for (int s = 32; s >= 1; s >>= 1) {
+                        const double ob = __shfl_xor(best, s, 64);
+                        const int oo = __shfl_xor(border, s, 64);
+                        if (ob < best || (ob == best && oo < border)) {
+                            best = ob;
+                            border = oo;
+                        }
+                    }
+                };
+                search();
+                // neighbour cells whose box is not farther than the best so far
+                uint32_t need = 0;
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
-                    const bool av = nb == CENTER ? (act && lane != ip) : (m[nb] != 0);
-                    if (!av) continue;
-                    if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
-                    const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
-                    const double dd = sqrt(du * du + dv * dv);
-                    if (dd < best) {
-                        best = dd;
-                        border = nb * 64 + lane;
-                    }
+                    if (nb == CENTER || ((loaded >> nb) & 1) || !o9[nb]) continue;
+                    const double gx = bl[nb][0] > pn[0] ? bl[nb][0] - pn[0]
+                                                        : (pn[0] > bh[nb][0] ? pn[0] - bh[nb][0] : 0.0);
+                    const double gy = bl[nb][1] > pn[1] ? bl[nb][1] - pn[1]
+                                                        : (pn[1] > bh[nb][1] ? pn[1] - bh[nb][1] : 0.0);
+                    if (!(sqrt(gx * gx + gy * gy) > best)) need |= 1u << nb;
                 }
+                if (need) {
 #pragma unroll
-                for (int s = 32; s >= 1; s >>= 1) {
-                    const double ob = __shfl_xor(best, s, 64);
-                    const int oo = __shfl_xor(border, s, 64);
-                    if (ob < best || (ob == best && oo < border)) {
-                        best = ob;
-                        border = oo;
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if (!((need >> nb) & 1)) continue;
+                        const bool a = (o9[nb] >> lane) & 1;
+                        const int64_t e2 = cn[nb] + C * lane;
+                        q[nb][0] = a ? coords.p[0][e2] : 0.0;
+                        q[nb][1] = a ? coords.p[1][e2] : 0.0;
                     }
+                    loaded |= need;
+                    search();
                 }
                 if (border == 0x7fffffff) {
                     ++n_orph;  // no neighbour at all: the field is left as it was
                 } else if (nargs) {
-                    const int nbw = border >> 6;  // uniform
-                    double mine = a[0];
+                    const int nbw = border >> 6, lw = border & 63;  // uniform
+                    double v;
+                    if (nbw == CENTER) {
+                        v = __shfl(a_c, lw, 64);
+                    } else {
+                        int64_t cs = cn[0];
 #pragma unroll
-                    for (int nb = 1; nb < NB; ++nb) mine = nb == nbw ? a[nb] : mine;
-                    const double v = __shfl(mine, border & 63, 64);
+                        for (int nb = 1; nb < NB; ++nb) cs = nb == nbw ? cn[nb] : cs;
+                        v = arg[cs + C * lw];
+                    }
                     if (lane == ip) {
-                        a[CENTER] = v;
+                        a_c = v;
                         arg[enew] = v;
                     }
                 }
                 ++n_inj;
                 if (num >= min_q) break;
             }
+        }
+        // later colour phases see this cell's final occupancy and box
+        const uint64_t fin = __ballot(act) & full;
+        if (lane == 0) {
+            occw[c] = fin;
+            box[c] = bl[CENTER][0];
+            box[C + c] = bh[CENTER][0];
+            box[2 * C + c] = bl[CENTER][1];
+            box[3 * C + c] = bh[CENTER][1];
         }
     }  // work list
     if (lane == 0) {
@@ -647,7 +734,7 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
     int64_t cap = 1;
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
-    if ((rc = inject_lists(ctx, G.C, NQ, cap, L))) return rc;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, D == 2 ? (size_t)G.C * 8 * 2 * D : 0, L))) return rc;
     k_inject_scan<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D), index,
                                                                   min_q, L);
     JP_LAUNCHED(ctx);
@@ -662,7 +749,8 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
             if (nargs <= 1) {
                 k_inject_wave2<<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
                     G, cp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
-                    L.list + (int64_t)ph * cap, L.total + ph, st);
+                    L.list + (int64_t)ph * cap, L.total + ph, L.occ,
+                    static_cast<double*>(L.box), st);
                 JP_LAUNCHED(ctx);
                 continue;
             }

@@ -455,6 +455,12 @@ __global__ void __launch_bounds__(kBlock) k_inj_scan_split(Grid<D> g, FPtr3 off,
     int num = 0;
     if (valid) {
         const uint64_t act = active_slots(index, c, g.C, g.K);
+        float bl[D], bh[D];  // box of the cell's offsets
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            bl[d] = __builtin_inff();
+            bh[d] = -__builtin_inff();
+        }
         constexpr int B = 16;
         for (int k0 = 0; k0 < g.K; k0 += B) {
             float ub[B][D];
@@ -466,10 +472,26 @@ __global__ void __launch_bounds__(kBlock) k_inj_scan_split(Grid<D> g, FPtr3 off,
             }
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                bool inside = (act >> (k0 + j)) & 1;
+                const bool on = (act >> (k0 + j)) & 1;
+                bool inside = on;
 #pragma unroll
-                for (int d = 0; d < D; ++d) inside = inside & (0.0f < ub[j][d] && ub[j][d] < 0.5f);
+                for (int d = 0; d < D; ++d) {
+                    inside = inside & (0.0f < ub[j][d] && ub[j][d] < 0.5f);
+                    if (D == 2 && on) {
+                        bl[d] = fminf(bl[d], ub[j][d]);
+                        bh[d] = fmaxf(bh[d], ub[j][d]);
+                    }
+                }
                 num += inside ? 1 : 0;
+            }
+        }
+        L.occ[c] = act;
+        if (D == 2) {
+            float* __restrict__ bx = static_cast<float*>(L.box);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                bx[(2 * d) * g.C + c] = bl[d];
+                bx[(2 * d + 1) * g.C + c] = bh[d];
             }
         }
     }
@@ -625,15 +647,20 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave_split(Grid<D> g, SGrid<D> s
     }
 }
 
-// 2D with at most one field: the 3x3 neighbourhood (mask, offsets, field) is
-// loaded once per listed cell into registers; nearest-neighbour copies come
-// from registers through a shuffle.
+// 2D with at most one field, loading only what the nearest-neighbour search
+// can use: the cell's own particles, and a neighbour cell's particles only
+// when the distance from the new particle to the box of that cell's offsets
+// (same floating point operations as the distances) does not exceed the best
+// distance so far.  Loaded neighbours stay in registers for the cell's later
+// injections; copies come from registers or one uniform load.
 __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> sg, FPtr3 off,
                                                             uint8_t* index, int nargs,
                                                             double* __restrict__ arg, int min_q,
                                                             uint64_t seed, uint64_t counter,
                                                             const int32_t* __restrict__ list,
                                                             const int32_t* __restrict__ count,
+                                                            uint64_t* __restrict__ occw,
+                                                            float* __restrict__ box,
                                                             jp_stats* st) {
     constexpr int NB = 9, CENTER = 4;
     const int lane = threadIdx.x & 63;
@@ -642,7 +669,6 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
     const int64_t C = g.C;
     const int K = g.K;
     const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
-    const bool own = lane < K;
     const double h0 = sg.h[0][0], h1 = sg.h[1][0];
     int64_t n_inj = 0, n_orph = 0;
     for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
@@ -650,20 +676,30 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
         const int64_t c = __builtin_amdgcn_readfirstlane(list[w]);
         int64_t I[2];
         unlin<2>(g.ni, c, I);
-        uint8_t m[NB];
-        float uq[NB][2];
-        double a[NB];
+        int64_t cn[NB];
+        uint64_t o9[NB];
+        float bl[NB][2], bh[NB][2];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
-            const bool ok = own && J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
-            const int64_t e2 = J0 + g.ni[0] * J1 + C * lane;
-            m[nb] = ok ? index[e2] : (uint8_t)0;
-            uq[nb][0] = ok ? off.p[0][e2] : 0.0f;
-            uq[nb][1] = ok ? off.p[1][e2] : 0.0f;
-            a[nb] = (ok && nargs) ? arg[e2] : 0.0;
+            const bool ok = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            cn[nb] = ok ? J0 + g.ni[0] * J1 : 0;
+            o9[nb] = ok ? occw[cn[nb]] : 0ull;
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                bl[nb][d] = ok ? box[(2 * d) * C + cn[nb]] : 0.0f;
+                bh[nb][d] = ok ? box[(2 * d + 1) * C + cn[nb]] : 0.0f;
+            }
         }
-        bool act = m[CENTER] != 0;
+        const int64_t e = c + C * lane;
+        bool act = (o9[CENTER] >> lane) & 1;
+        float uq[NB][2];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) uq[nb][0] = uq[nb][1] = 0.0f;
+        uq[CENTER][0] = act ? off.p[0][e] : 0.0f;
+        uq[CENTER][1] = act ? off.p[1][e] : 0.0f;
+        double a_c = (act && nargs) ? arg[e] : 0.0;
+        uint32_t loaded = 1u << CENTER;
         for (int qd = 0; qd < 4; ++qd) {
             const float q0 = 0.5f * (float)(qd & 1), q1 = 0.5f * (float)((qd >> 1) & 1);
             const bool inside = act && (q0 < uq[CENTER][0] && uq[CENTER][0] < q0 + 0.5f) &&
@@ -676,9 +712,15 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                 freeS &= freeS - 1;
                 ++num;
                 float un[2];
-                un[0] = (float)((double)q0 + 0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 0, 2, counter), 0.05));
-                un[1] = (float)((double)q1 + 0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 1, 2, counter), 0.05));
+                un[0] = (float)((double)q0 +
+                                0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 0, 2, counter), 0.05));
+                un[1] = (float)((double)q1 +
+                                0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 1, 2, counter), 0.05));
                 const int64_t enew = c + C * ip;
+                bl[CENTER][0] = fminf(bl[CENTER][0], un[0]);
+                bh[CENTER][0] = fmaxf(bh[CENTER][0], un[0]);
+                bl[CENTER][1] = fminf(bl[CENTER][1], un[1]);
+                bh[CENTER][1] = fmaxf(bh[CENTER][1], un[1]);
                 if (lane == ip) {
                     act = true;
                     uq[CENTER][0] = un[0];
@@ -687,47 +729,93 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                     off.p[1][enew] = un[1];
                     index[enew] = 1;
                 }
-                double best = __builtin_inf();
-                int border = 0x7fffffff;
+                double best;
+                int border;
+                auto search = [&]() __attribute__((always_inline)) {
+                    best = __builtin_inf();
+                    border = 0x7fffffff;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if (!((loaded >> nb) & 1)) continue;
+                        const bool av = nb == CENTER ? (act && lane != ip) : ((o9[nb] >> lane) & 1);
+                        if (!av) continue;
+                        const double r0 =
+                            ((double)(nb % 3 - 1) + (double)uq[nb][0] - (double)un[0]) * h0;
+                        const double r1 =
+                            ((double)(nb / 3 - 1) + (double)uq[nb][1] - (double)un[1]) * h1;
+                        const double dd = sqrt(r0 * r0 + r1 * r1);
+                        if (dd < best) {
+                            best = dd;
+                            border = nb * 64 + lane;
+                        }
+                    }
+#pragma unroll
+                    for (int s = 32; s >= 1; s >>= 1) {
+                        const double ob = __shfl_xor(best, s, 64);
+                        const int oo = __shfl_xor(border, s, 64);
+                        if (ob < best || (ob == best && oo < border)) {
+                            best = ob;
+                            border = oo;
+                        }
+                    }
+                };
+                search();
+                uint32_t need = 0;
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
-                    const bool av = nb == CENTER ? (act && lane != ip) : (m[nb] != 0);
-                    if (!av) continue;
-                    const double r0 =
-                        ((double)(nb % 3 - 1) + (double)uq[nb][0] - (double)un[0]) * h0;
-                    const double r1 =
-                        ((double)(nb / 3 - 1) + (double)uq[nb][1] - (double)un[1]) * h1;
-                    const double dd = sqrt(r0 * r0 + r1 * r1);
-                    if (dd < best) {
-                        best = dd;
-                        border = nb * 64 + lane;
-                    }
+                    if (nb == CENTER || ((loaded >> nb) & 1) || !o9[nb]) continue;
+                    // r(u) = ((o + u) - un) * h is monotonic in u: its smallest
+                    // magnitude over [lo, hi] bounds every particle of the cell
+                    const double lo0 = ((double)(nb % 3 - 1) + (double)bl[nb][0] - (double)un[0]) * h0;
+                    const double hi0 = ((double)(nb % 3 - 1) + (double)bh[nb][0] - (double)un[0]) * h0;
+                    const double lo1 = ((double)(nb / 3 - 1) + (double)bl[nb][1] - (double)un[1]) * h1;
+                    const double hi1 = ((double)(nb / 3 - 1) + (double)bh[nb][1] - (double)un[1]) * h1;
+                    const double gx = lo0 > 0.0 ? lo0 : (hi0 < 0.0 ? hi0 : 0.0);
+                    const double gy = lo1 > 0.0 ? lo1 : (hi1 < 0.0 ? hi1 : 0.0);
+                    if (!(sqrt(gx * gx + gy * gy) > best)) need |= 1u << nb;
                 }
+                if (need) {
 #pragma unroll
-                for (int s = 32; s >= 1; s >>= 1) {
-                    const double ob = __shfl_xor(best, s, 64);
-                    const int oo = __shfl_xor(border, s, 64);
-                    if (ob < best || (ob == best && oo < border)) {
-                        best = ob;
-                        border = oo;
+                    for (int nb = 0; nb < NB; ++nb) {
+                        if (!((need >> nb) & 1)) continue;
+                        const bool a = (o9[nb] >> lane) & 1;
+                        const int64_t e2 = cn[nb] + C * lane;
+                        uq[nb][0] = a ? off.p[0][e2] : 0.0f;
+                        uq[nb][1] = a ? off.p[1][e2] : 0.0f;
                     }
+                    loaded |= need;
+                    search();
                 }
                 if (border == 0x7fffffff) {
                     ++n_orph;
                 } else if (nargs) {
-                    const int nbw = border >> 6;
-                    double mine = a[0];
+                    const int nbw = border >> 6, lw = border & 63;
+                    double v;
+                    if (nbw == CENTER) {
+                        v = __shfl(a_c, lw, 64);
+                    } else {
+                        int64_t cs = cn[0];
 #pragma unroll
-                    for (int nb = 1; nb < NB; ++nb) mine = nb == nbw ? a[nb] : mine;
-                    const double v = __shfl(mine, border & 63, 64);
+                        for (int nb = 1; nb < NB; ++nb) cs = nb == nbw ? cn[nb] : cs;
+                        v = arg[cs + C * lw];
+                    }
                     if (lane == ip) {
-                        a[CENTER] = v;
+                        a_c = v;
                         arg[enew] = v;
                     }
                 }
                 ++n_inj;
                 if (num >= min_q) break;
             }
+        }
+        // later colour phases see this cell's final occupancy and box
+        const uint64_t fin = __ballot(act) & full;
+        if (lane == 0) {
+            occw[c] = fin;
+            box[c] = bl[CENTER][0];
+            box[C + c] = bh[CENTER][0];
+            box[2 * C + c] = bl[CENTER][1];
+            box[3 * C + c] = bh[CENTER][1];
         }
     }  // work list
     if (lane == 0) {
@@ -830,7 +918,7 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     int64_t cap = 1;
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
-    if ((rc = inject_lists(ctx, G.C, NQ, cap, L))) return rc;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, D == 2 ? (size_t)G.C * 4 * 2 * D : 0, L))) return rc;
     FPtr3 fp = fptr3<D>(off);
     k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, L);
     JP_LAUNCHED(ctx);
@@ -843,7 +931,8 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
             if (nargs <= 1) {
                 k_inj_wave2_split<<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
                     G, sg, fp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
-                    L.list + (int64_t)ph * cap, L.total + ph, st);
+                    L.list + (int64_t)ph * cap, L.total + ph, L.occ,
+                    static_cast<float*>(L.box), st);
                 JP_LAUNCHED(ctx);
                 continue;
             }
