@@ -216,13 +216,18 @@ __global__ void __launch_bounds__(kBlock) k_advect(Grid<D> g, Vel<D> vg, VPtrs<D
         }
     }
     const int tid = threadIdx.x;
+    // window fills: every load issued before the LDS stores
+    constexpr int NI = (win_size<D>() + kBlock - 1) / kBlock;
+    double wv[D][NI];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         const double* __restrict__ F = V.v[c];
-        for (int i = tid; i < win_size<D>(); i += kBlock) {
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+            const int i = tid + it * kBlock;
             int r = i;
             int64_t node = 0, stride = 1;
-            bool ok = true;
+            bool ok = i < win_size<D>();
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int64_t k = base[d] + r % win_n<D>(d);
@@ -231,20 +236,41 @@ __global__ void __launch_bounds__(kBlock) k_advect(Grid<D> g, Vel<D> vg, VPtrs<D
                 node += k * stride;
                 stride *= vg.ax[c][d].n;
             }
-            L.V[c][i] = ok ? F[node] : 0.0;
+            wv[c][it] = ok ? F[ok ? node : 0] : 0.0;
         }
     }
     constexpr int NA = D * D * win_max<D>();
-    for (int i = tid; i < NA; i += kBlock) {
+    constexpr int NJ = (NA + kBlock - 1) / kBlock;
+    double ax[NJ], ai[NJ];
+#pragma unroll
+    for (int it = 0; it < NJ; ++it) {
+        const int i = tid + it * kBlock;
         const int c = i / (D * win_max<D>());
         const int d = (i / win_max<D>()) % D;
         const int j = i % win_max<D>();
-        const Axis& A = vg.ax[c][d];
-        const int64_t k = base[d] + j;
-        const bool okx = j < win_n<D>(d) && k >= 0 && k < A.n;
-        const bool oki = j < win_n<D>(d) && k >= 0 && k < A.n - 1;
-        L.x[c][d][j] = okx ? A.x[k] : __builtin_nan("");
-        L.idx[c][d][j] = oki ? A.idx[k] : __builtin_nan("");
+        const bool in = i < NA;
+        const Axis& A = vg.ax[in ? c : 0][in ? d : 0];
+        const int64_t k = base[in ? d : 0] + j;
+        const bool okx = in && j < win_n<D>(d) && k >= 0 && k < A.n;
+        const bool oki = in && j < win_n<D>(d) && k >= 0 && k < A.n - 1;
+        ax[it] = okx ? A.x[okx ? k : 0] : __builtin_nan("");
+        ai[it] = oki ? A.idx[oki ? k : 0] : __builtin_nan("");
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int it = 0; it < NI; ++it)
+            if (tid + it * kBlock < win_size<D>()) L.V[c][tid + it * kBlock] = wv[c][it];
+#pragma unroll
+    for (int it = 0; it < NJ; ++it) {
+        const int i = tid + it * kBlock;
+        if (i < NA) {
+            const int c = i / (D * win_max<D>());
+            const int d = (i / win_max<D>()) % D;
+            const int j = i % win_max<D>();
+            L.x[c][d][j] = ax[it];
+            L.idx[c][d][j] = ai[it];
+        }
     }
     if (tid < D * D) {
         const int c = tid / D, d = tid % D;

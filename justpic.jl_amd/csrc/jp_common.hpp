@@ -168,18 +168,26 @@ __device__ inline bool row_cell(const Grid<D>& g, int64_t& c, int64_t (&I)[D]) {
     return true;
 }
 
-// Active slots of cell c as a bitmask: the K mask bytes are loaded in
-// unrolled chunks of 8 independent loads (coalesced across neighbouring
+// Active slots of cell c as a bitmask (loads coalesced across neighbouring
 // cells of a wavefront in the cell-plane layout).
 __device__ inline uint64_t active_slots(const uint8_t* __restrict__ index, int64_t c, int64_t C,
                                         int K) {
+    // 32 loads per round, issued together: the slot index is clamped to K-1
+    // (always a valid address) instead of guarding each load, since a guarded
+    // load is waited for on its own
     uint64_t bits = 0;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        uint8_t m[16];
 #pragma unroll
-        for (int v = 0; v < 16; ++v) m[v] = (k0 + v < K) ? index[c + C * (k0 + v)] : (uint8_t)0;
+    for (int k0 = 0; k0 < 64; k0 += 32) {
+        if (k0 >= K) break;
+        uint8_t m[32];
 #pragma unroll
-        for (int v = 0; v < 16; ++v) bits |= (uint64_t)(m[v] != 0) << (k0 + v);
+        for (int v = 0; v < 32; ++v) {
+            const int k = k0 + v < K ? k0 + v : K - 1;
+            m[v] = index[c + C * k];
+        }
+#pragma unroll
+        for (int v = 0; v < 32; ++v)
+            bits |= (uint64_t)(m[v] != 0 && k0 + v < K) << (k0 + v);
     }
     return bits;
 }

@@ -495,18 +495,25 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
         int64_t cn[NB];
         uint64_t o9[NB];
         double bl[NB][2], bh[NB][2];
+        bool okn[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
-            const bool ok = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
-            cn[nb] = ok ? J0 + g.ni[0] * J1 : 0;
-            o9[nb] = ok ? occw[cn[nb]] : 0ull;
+            okn[nb] = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            cn[nb] = okn[nb] ? J0 + g.ni[0] * J1 : c;  // a valid cell either way
+        }
+        // unguarded loads, all issued before any is used; masked afterwards
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            o9[nb] = occw[cn[nb]];
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
-                bl[nb][d] = ok ? box[(2 * d) * C + cn[nb]] : 0.0;
-                bh[nb][d] = ok ? box[(2 * d + 1) * C + cn[nb]] : 0.0;
+                bl[nb][d] = box[(2 * d) * C + cn[nb]];
+                bh[nb][d] = box[(2 * d + 1) * C + cn[nb]];
             }
         }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
         const int64_t e = c + C * lane;
         bool act = (o9[CENTER] >> lane) & 1;
         double q[NB][2];
@@ -515,6 +522,9 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
         q[CENTER][0] = act ? coords.p[0][e] : 0.0;
         q[CENTER][1] = act ? coords.p[1][e] : 0.0;
         double a_c = (act && nargs) ? arg[e] : 0.0;
+        // this cell's new particles stay in registers until the cell is done
+        // (a load issued after a store would also wait for the store)
+        bool inj = false, copied = false;
         uint32_t loaded = 1u << CENTER;
         for (int qd = 0; qd < 4; ++qd) {
             double vert[2];
@@ -537,18 +547,15 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                     const double r = uniform53(seed, (uint64_t)c, ip, d, 2, counter);
                     pn[d] = vert[d] + dq[d] * fma(0.95, r, 0.05);
                 }
-                const int64_t enew = c + C * ip;
                 bl[CENTER][0] = fmin(bl[CENTER][0], pn[0]);
                 bh[CENTER][0] = fmax(bh[CENTER][0], pn[0]);
                 bl[CENTER][1] = fmin(bl[CENTER][1], pn[1]);
                 bh[CENTER][1] = fmax(bh[CENTER][1], pn[1]);
                 if (lane == ip) {
                     act = true;
+                    inj = true;
                     q[CENTER][0] = pn[0];
                     q[CENTER][1] = pn[1];
-                    coords.p[0][enew] = pn[0];
-                    coords.p[1][enew] = pn[1];
-                    index[enew] = 1;
                 }
                 // index_min_distance over the loaded cells: first occurrence
                 // of the minimum in (cell, slot) order
@@ -593,13 +600,15 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                     if (!(sqrt(gx * gx + gy * gy) > best)) need |= 1u << nb;
                 }
                 if (need) {
+                    // one predicate per lane (no uniform branch per cell), so the
+                    // loads of all needed cells are in flight together
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) {
-                        if (!((need >> nb) & 1)) continue;
-                        const bool a = (o9[nb] >> lane) & 1;
+                        if (nb == CENTER) continue;
+                        const bool take = ((need >> nb) & 1) && ((o9[nb] >> lane) & 1);
                         const int64_t e2 = cn[nb] + C * lane;
-                        q[nb][0] = a ? coords.p[0][e2] : 0.0;
-                        q[nb][1] = a ? coords.p[1][e2] : 0.0;
+                        q[nb][0] = take ? coords.p[0][e2] : q[nb][0];
+                        q[nb][1] = take ? coords.p[1][e2] : q[nb][1];
                     }
                     loaded |= need;
                     search();
@@ -619,12 +628,18 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                     }
                     if (lane == ip) {
                         a_c = v;
-                        arg[enew] = v;
+                        copied = true;
                     }
                 }
                 ++n_inj;
                 if (num >= min_q) break;
             }
+        }
+        if (inj) {
+            coords.p[0][e] = q[CENTER][0];
+            coords.p[1][e] = q[CENTER][1];
+            index[e] = 1;
+            if (copied) arg[e] = a_c;
         }
         // later colour phases see this cell's final occupancy and box
         const uint64_t fin = __ballot(act) & full;

@@ -236,13 +236,18 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
 #pragma unroll
     for (int c = 0; c < D; ++c) sc[c] = dt / g.h[c][0];
     const int tid = threadIdx.x;
+    // window fill: every load issued before the LDS stores
+    constexpr int NI = (win_size<D>() + kBlock - 1) / kBlock;
+    float wv[D][NI];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         const double* __restrict__ F = V.v[c];
-        for (int i = tid; i < win_size<D>(); i += kBlock) {
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+            const int i = tid + it * kBlock;
             int r = i;
             int64_t node = 0, stride = 1;
-            bool ok = true;
+            bool ok = i < win_size<D>();
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int k = t0[d] - 1 + r % win_n<D>(d);
@@ -251,9 +256,14 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
                 node += k * stride;
                 stride *= V.n[c][d];
             }
-            sV[c][i] = ok ? (float)(F[node] * sc[c]) : 0.0f;
+            wv[c][it] = ok ? (float)(F[ok ? node : 0] * sc[c]) : 0.0f;
         }
     }
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int it = 0; it < NI; ++it)
+            if (tid + it * kBlock < win_size<D>()) sV[c][tid + it * kBlock] = wv[c][it];
     __syncthreads();
     int I[D];
     bool in = true;
@@ -679,18 +689,25 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
         int64_t cn[NB];
         uint64_t o9[NB];
         float bl[NB][2], bh[NB][2];
+        bool okn[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
-            const bool ok = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
-            cn[nb] = ok ? J0 + g.ni[0] * J1 : 0;
-            o9[nb] = ok ? occw[cn[nb]] : 0ull;
+            okn[nb] = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            cn[nb] = okn[nb] ? J0 + g.ni[0] * J1 : c;  // a valid cell either way
+        }
+        // unguarded loads, all issued before any is used; masked afterwards
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            o9[nb] = occw[cn[nb]];
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
-                bl[nb][d] = ok ? box[(2 * d) * C + cn[nb]] : 0.0f;
-                bh[nb][d] = ok ? box[(2 * d + 1) * C + cn[nb]] : 0.0f;
+                bl[nb][d] = box[(2 * d) * C + cn[nb]];
+                bh[nb][d] = box[(2 * d + 1) * C + cn[nb]];
             }
         }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
         const int64_t e = c + C * lane;
         bool act = (o9[CENTER] >> lane) & 1;
         float uq[NB][2];
@@ -699,6 +716,9 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
         uq[CENTER][0] = act ? off.p[0][e] : 0.0f;
         uq[CENTER][1] = act ? off.p[1][e] : 0.0f;
         double a_c = (act && nargs) ? arg[e] : 0.0;
+        // this cell's new particles stay in registers until the cell is done
+        // (a load issued after a store would also wait for the store)
+        bool inj = false, copied = false;
         uint32_t loaded = 1u << CENTER;
         for (int qd = 0; qd < 4; ++qd) {
             const float q0 = 0.5f * (float)(qd & 1), q1 = 0.5f * (float)((qd >> 1) & 1);
@@ -716,18 +736,15 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                                 0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 0, 2, counter), 0.05));
                 un[1] = (float)((double)q1 +
                                 0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 1, 2, counter), 0.05));
-                const int64_t enew = c + C * ip;
                 bl[CENTER][0] = fminf(bl[CENTER][0], un[0]);
                 bh[CENTER][0] = fmaxf(bh[CENTER][0], un[0]);
                 bl[CENTER][1] = fminf(bl[CENTER][1], un[1]);
                 bh[CENTER][1] = fmaxf(bh[CENTER][1], un[1]);
                 if (lane == ip) {
                     act = true;
+                    inj = true;
                     uq[CENTER][0] = un[0];
                     uq[CENTER][1] = un[1];
-                    off.p[0][enew] = un[0];
-                    off.p[1][enew] = un[1];
-                    index[enew] = 1;
                 }
                 double best;
                 int border;
@@ -775,13 +792,15 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                     if (!(sqrt(gx * gx + gy * gy) > best)) need |= 1u << nb;
                 }
                 if (need) {
+                    // one predicate per lane (no uniform branch per cell), so the
+                    // loads of all needed cells are in flight together
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) {
-                        if (!((need >> nb) & 1)) continue;
-                        const bool a = (o9[nb] >> lane) & 1;
+                        if (nb == CENTER) continue;
+                        const bool take = ((need >> nb) & 1) && ((o9[nb] >> lane) & 1);
                         const int64_t e2 = cn[nb] + C * lane;
-                        uq[nb][0] = a ? off.p[0][e2] : 0.0f;
-                        uq[nb][1] = a ? off.p[1][e2] : 0.0f;
+                        uq[nb][0] = take ? off.p[0][e2] : uq[nb][0];
+                        uq[nb][1] = take ? off.p[1][e2] : uq[nb][1];
                     }
                     loaded |= need;
                     search();
@@ -801,12 +820,18 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                     }
                     if (lane == ip) {
                         a_c = v;
-                        arg[enew] = v;
+                        copied = true;
                     }
                 }
                 ++n_inj;
                 if (num >= min_q) break;
             }
+        }
+        if (inj) {
+            off.p[0][e] = uq[CENTER][0];
+            off.p[1][e] = uq[CENTER][1];
+            index[e] = 1;
+            if (copied) arg[e] = a_c;
         }
         // later colour phases see this cell's final occupancy and box
         const uint64_t fin = __ballot(act) & full;
