@@ -312,8 +312,9 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
 }
 
 // ------------------------------------------------------------ particle2grid
-// Inverse-distance-squared weights in float32 from the offsets, weight and
-// weighted-field sums in FP64; cells and slots in the reference order.
+// Inverse-distance-squared weights in float32 from the offsets (hardware
+// reciprocal, about 1 ulp), weight and weighted-field sums in FP64; cells
+// and slots in the reference order.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_p2g_split(Grid<D> gg, SGrid<D> g, double* __restrict__ F,
                                                       const double* __restrict__ Fp, FPtr3 off,
@@ -356,67 +357,77 @@ __global__ void __launch_bounds__(kBlock) k_p2g_split(Grid<D> gg, SGrid<D> g, do
         bits[q] = ok ? occw[cell[q]] : 0ull;
     }
     double W = 0.0, WF = 0.0;
+    // node minus cell corner in cells (the node is corner 1 - bit of cell q)
     constexpr int B = D == 2 ? 8 : 4;
     const int nk = (g.K + B - 1) / B;
-    const int total = NC * nk;
-    auto chunk = [&](int t, int64_t& c, uint32_t& bq, int& k0, int& q) __attribute__((always_inline)) {
-        q = t / nk;
-        k0 = (t - q * nk) * B;
-        c = cell[0];
+    // the (cell, chunk) walk with the next chunk's loads in flight
+    auto load = [&](int q, int k0, float (&ub)[B][D], double (&fb)[B]) __attribute__((always_inline)) {
+        int64_t c = cell[0];
         uint64_t bw = bits[0];
 #pragma unroll
         for (int u = 1; u < NC; ++u) {
             c = q == u ? cell[u] : c;
             bw = q == u ? bits[u] : bw;
         }
-        bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
-    };
-    auto load = [&](int t, float (&ub)[B][D], double (&fb)[B]) __attribute__((always_inline)) {
-        int64_t c;
-        uint32_t bq;
-        int k0, q;
-        chunk(t, c, bq, k0, q);
+        const uint32_t bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (bq >> j) & 1;
             const int64_t e = c + C * (k0 + j);
 #pragma unroll
-            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][e] : 0.0f;
+            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][e] : 2.0f;
             fb[j] = on ? Fp[e] : 0.0;
         }
     };
-    auto accumulate = [&](int t, const float (&ub)[B][D], const double (&fb)[B])
+    // a weight from an inactive slot (offset 2 in every dimension, so s > 0)
+    // is added as zero: the accumulation is branch-free
+    auto accumulate = [&](int q, int k0, const float (&ub)[B][D], const double (&fb)[B])
                           __attribute__((always_inline)) {
-        int64_t c;
-        uint32_t bq;
-        int k0, q;
-        chunk(t, c, bq, k0, q);
+        uint64_t bw = bits[0];
+#pragma unroll
+        for (int u = 1; u < NC; ++u) bw = q == u ? bits[u] : bw;
+        const uint32_t bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
+        float a[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[d] = ((q >> d) & 1) ? 0.0f : 1.0f;
 #pragma unroll
         for (int j = 0; j < B; ++j) {
-            if (!((bq >> j) & 1)) continue;
             float s = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                // node minus particle in cells: the node is corner 1 - bit of the cell
-                const float a = (((q >> d) & 1) ? 0.0f : 1.0f) - ub[j][d];
-                const float r = a * hf[d];
+                const float r = (a[d] - ub[j][d]) * hf[d];
                 s = fmaf(r, r, s);
             }
-            const double w = (double)(1.0f / s);
+            const double w = ((bq >> j) & 1) ? (double)__builtin_amdgcn_rcpf(s) : 0.0;
             W += w;
             WF = fma(w, fb[j], WF);
         }
     };
     float u0[B][D], u1[B][D];
     double f0[B], f1[B];
-    load(0, u0, f0);
+    int q = 0, k0 = 0;
+    load(0, 0, u0, f0);
+    const int total = NC * nk;
 #pragma unroll 1
     for (int t = 0; t < total; t += 2) {
-        if (t + 1 < total) load(t + 1, u1, f1);
-        accumulate(t, u0, f0);
+        // next chunk: (q, k0 + B), wrapping to the next cell
+        int q1 = q, k1 = k0 + B;
+        if (k1 >= nk * B) {
+            q1 = q + 1;
+            k1 = 0;
+        }
+        if (t + 1 < total) load(q1, k1, u1, f1);
+        accumulate(q, k0, u0, f0);
         if (t + 1 >= total) break;
-        if (t + 2 < total) load(t + 2, u0, f0);
-        accumulate(t + 1, u1, f1);
+        int q2 = q1, k2 = k1 + B;
+        if (k2 >= nk * B) {
+            q2 = q1 + 1;
+            k2 = 0;
+        }
+        if (t + 2 < total) load(q2, k2, u0, f0);
+        accumulate(q1, k1, u1, f1);
+        q = q2;
+        k0 = k2;
     }
     F[node] = WF / W;
 }
