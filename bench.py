@@ -74,6 +74,18 @@ def cpu_baseline(cfg, n_steps, seed):
 CALLS = ("advection", "move", "inject", "p2g")
 
 
+def read_traffic(path):
+    """HBM bytes per launch of the advection kernel from a committed counter
+    summary (a line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_form(jp, ctx, cfg, form, args):
     """Warm up, then time args.steps full steps of one position form.  Every
     call is bracketed by hipEvents on the context stream."""
@@ -196,6 +208,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the other form and the error check")
     ap.add_argument("--check-steps", type=int, default=50)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--cpu-steps", type=int, default=6,
                     help="oracle steps timed for the CPU baseline (0 = skip)")
     args = ap.parse_args()
@@ -206,6 +221,17 @@ def main():
 
     import justpic_hip as jp
     from justpic_hip.configs import stream2d
+
+    if args.calibrate_copy > 0:
+        ctx = jp.Context(0)
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = jp.DeviceArray(ctx, (n,)).fill(1.0)
+        b = jp.DeviceArray(ctx, (n,))
+        for _ in range(5):
+            b.copy_from_device(a)
+        ctx.sync()
+        log(f"copied {args.calibrate_copy} MiB five times")
+        return
 
     log("building workload", args.n)
     cfg = stream2d(args.n, amp=250.0, readme=False)
@@ -222,13 +248,8 @@ def main():
 
     # roofline: the advection kernel (one launch per call)
     adv = main_r["calls"]["advection"]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_advection_{args.form}_{args.n}.json")
-    if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = read_traffic(os.path.join(ROOT, "profiles",
+                                        f"traffic_advection_{args.form}_{args.n}.txt"))
 
     cpu = None
     if args.cpu_steps > 0:

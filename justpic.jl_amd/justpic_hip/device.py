@@ -139,8 +139,13 @@ class DeviceArray:
 
     def copy(self):
         o = DeviceArray(self.ctx, self.shape, self.dtype)
-        self.ctx.check(self.ctx.lib.jp_memcpy_d2d(self.ctx.h, o.ptr, self.ptr, self.nbytes), "d2d")
+        o.copy_from_device(self)
         return o
+
+    def copy_from_device(self, src: "DeviceArray"):
+        if src.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {src.nbytes} vs {self.nbytes} bytes")
+        self.ctx.check(self.ctx.lib.jp_memcpy_d2d(self.ctx.h, self.ptr, src.ptr, self.nbytes), "d2d")
 
     def free(self):
         if self.ptr:
