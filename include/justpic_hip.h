@@ -195,6 +195,28 @@ int jp_inject_particles_split(jp_ctx* ctx, const jp_cellgrid* g, float* const* o
                               int min_xcell, uint64_t seed, uint64_t call_counter,
                               jp_stats* stats);
 
+/* ---- batched ensemble (no reference counterpart) -----------------------
+ * n independent 2D grids in the FP32-split form, advected in one persistent
+ * launch: the tiles of all grids form one work list walked by a fixed grid
+ * of workgroups.  Grid i uses grids[i], vgs[i] (standard staggering),
+ * offsets off[2i], off[2i+1], mask index[i], velocities V[2i], V[2i+1] and
+ * time step dt[i].  The pointers are stored: the arrays must stay allocated
+ * until jp_ensemble_destroy.  Eight launch variants (tile shape 64x4, 32x8,
+ * 16x16, 128x2 cells x 2 or 8 workgroups per CU); jp_ensemble_tune times
+ * them on the ensemble itself (with dt = 0, so no particle moves) and keeps
+ * the fastest.  jp_ensemble_advect can be captured in a hipGraph together
+ * with the per-grid move and particle2grid calls of a time step. */
+#define JP_ENS_VARIANTS 8
+typedef struct jp_ensemble jp_ensemble;
+int jp_ensemble_create(jp_ctx* ctx, int n, const jp_cellgrid* grids, const jp_velgrid* vgs,
+                       float* const* off, uint8_t* const* index, const double* const* V,
+                       const double* dt, int method, double alpha, jp_ensemble** out);
+int jp_ensemble_advect(jp_ctx* ctx, jp_ensemble* ens);
+int jp_ensemble_set_variant(jp_ctx* ctx, jp_ensemble* ens, int variant);
+/* ms (optional, JP_ENS_VARIANTS floats): average launch time per variant */
+int jp_ensemble_tune(jp_ctx* ctx, jp_ensemble* ens, int reps, int* best, float* ms);
+int jp_ensemble_destroy(jp_ctx* ctx, jp_ensemble* ens);
+
 /* Philox4x32-10 block (host helper, for tests): out[4] <- philox(ctr[4], key[2]) */
 void jp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -11,6 +11,10 @@
 // (README.md:42-44).
 #include "jp_ctx.hpp"
 
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
 using namespace jp;
 
 namespace {
@@ -89,18 +93,19 @@ __global__ void __launch_bounds__(kBlock) k_from_split(Grid<D> g, FPtr3 u, const
 // points further out read the same values from global memory.  Each thread
 // walks its cell's slots in chunks of 8 planes, in lockstep with the
 // neighbouring cells of its wavefront.
+// Tile shapes: TX*TY*TZ = kBlock cells, one per thread; the LDS window holds
+// nodes [t0 - 1, t0 + T + 3) per dimension.
+template <int D, int TX, int TY, int TZ = 1>
+struct Tile {
+    static_assert(TX * TY * TZ == kBlock, "one thread per cell of the tile");
+    __host__ __device__ static constexpr int n(int d) { return d == 0 ? TX : (d == 1 ? TY : TZ); }
+    __host__ __device__ static constexpr int w(int d) { return n(d) + 4; }
+    __host__ __device__ static constexpr int size() {
+        return D == 2 ? (TX + 4) * (TY + 4) : (TX + 4) * (TY + 4) * (TZ + 4);
+    }
+};
 template <int D>
-__host__ __device__ constexpr int tile_n(int d) {
-    return D == 2 ? (d == 0 ? 64 : 4) : (d == 0 ? 16 : 4);
-}
-template <int D>
-__host__ __device__ constexpr int win_n(int d) {
-    return tile_n<D>(d) + 4;
-}
-template <int D>
-__host__ __device__ constexpr int win_size() {
-    return D == 2 ? win_n<2>(0) * win_n<2>(1) : win_n<3>(0) * win_n<3>(1) * win_n<3>(2);
-}
+using DefaultTile = typename std::conditional<D == 2, Tile<2, 64, 4>, Tile<3, 16, 4, 4>>::type;
 
 // FP64 input velocity (component grids n[c][d]) and the vertex spacings
 template <int D>
@@ -111,7 +116,7 @@ struct SVelIn {
 
 // velocity of component c at cell I + offset u (u in cell units, any value),
 // from the LDS window sw (origin t0 - 1) when it holds the four/eight corners
-template <int D>
+template <int D, class TS>
 __device__ inline float interp_tile(const float* __restrict__ sw, const SVelIn<D>& V,
                                     const SGrid<D>& g, int c, double sc, const int (&t0)[D],
                                     const int (&I)[D], const float (&u)[D]) {
@@ -135,11 +140,11 @@ __device__ inline float interp_tile(const float* __restrict__ sw, const SVelIn<D
             t[d] = up ? f - 0.5f : f + 0.5f;
         }
         const int kl = k[d] - (t0[d] - 1);
-        inw = inw && kl >= 0 && kl + 1 < win_n<D>(d);
+        inw = inw && kl >= 0 && kl + 1 < TS::w(d);
     }
     float q[1 << D];
     if (inw) {
-        constexpr int W0 = win_n<D>(0), W1 = win_n<D>(1);
+        constexpr int W0 = TS::w(0), W1 = TS::w(1);
         int b = (k[0] - t0[0] + 1) + W0 * (k[1] - t0[1] + 1);
         if constexpr (D == 3) b += W0 * W1 * (k[2] - t0[2] + 1);
 #pragma unroll
@@ -173,14 +178,14 @@ struct RKf {
     float w0, w1;
 };
 
-template <int D, int M>
-__device__ inline void advect_split_one(const float (*sV)[win_size<D>()], const SVelIn<D>& V,
+template <int D, int M, class TS>
+__device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVelIn<D>& V,
                                         const SGrid<D>& g, const double (&sc)[D],
                                         const int (&t0)[D], const int (&I)[D], const RKf& P,
                                         const float (&u0)[D], float (&out)[D]) {
     float v0[D];
 #pragma unroll
-    for (int cc = 0; cc < D; ++cc) v0[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, u0);
+    for (int cc = 0; cc < D; ++cc) v0[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, u0);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = u0[d] + v0[d];
@@ -189,7 +194,7 @@ __device__ inline void advect_split_one(const float (*sV)[win_size<D>()], const 
 #pragma unroll
         for (int d = 0; d < D; ++d) u1[d] = fmaf(P.alpha, v0[d], u0[d]);
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) v1[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, u1);
+        for (int cc = 0; cc < D; ++cc) v1[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, u1);
         if (P.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = u0[d] + v1[d];
@@ -202,33 +207,34 @@ __device__ inline void advect_split_one(const float (*sV)[win_size<D>()], const 
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + 0.5f * v0[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k2[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, q);
+        for (int cc = 0; cc < D; ++cc) k2[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, q);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + 0.5f * k2[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k3[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, q);
+        for (int cc = 0; cc < D; ++cc) k3[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, q);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + k3[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k4[cc] = interp_tile<D>(sV[cc], V, g, cc, sc[cc], t0, I, q);
+        for (int cc = 0; cc < D; ++cc) k4[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, q);
 #pragma unroll
         for (int d = 0; d < D; ++d)
             out[d] = u0[d] + (((v0[d] + 2.0f * k2[d]) + 2.0f * k3[d]) + k4[d]) / 6.0f;
     }
 }
 
-template <int D, int M>
-__global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, double dt,
-                                                      FPtr3 off, const uint8_t* __restrict__ index,
-                                                      RKf P) {
-    __shared__ float sV[D][win_size<D>()];
+// advection of the particles of tile `tile` of one grid (all threads of the
+// block call it; the LDS window sV is the block's)
+template <int D, int M, class TS>
+__device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, double dt,
+                                      const FPtr3& off, const uint8_t* __restrict__ index,
+                                      const RKf& P, int tile, float (*sV)[TS::size()]) {
     int t0[D];
     {
-        int b = blockIdx.x;
+        int b = tile;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int nt = (g.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
-            t0[d] = (b % nt) * tile_n<D>(d);
+            const int nt = (g.ni[d] + TS::n(d) - 1) / TS::n(d);
+            t0[d] = (b % nt) * TS::n(d);
             b /= nt;
         }
     }
@@ -237,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
     for (int c = 0; c < D; ++c) sc[c] = dt / g.h[c][0];
     const int tid = threadIdx.x;
     // window fill: every load issued before the LDS stores
-    constexpr int NI = (win_size<D>() + kBlock - 1) / kBlock;
+    constexpr int NI = (TS::size() + kBlock - 1) / kBlock;
     float wv[D][NI];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -247,11 +253,11 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
             const int i = tid + it * kBlock;
             int r = i;
             int64_t node = 0, stride = 1;
-            bool ok = i < win_size<D>();
+            bool ok = i < TS::size();
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const int k = t0[d] - 1 + r % win_n<D>(d);
-                r /= win_n<D>(d);
+                const int k = t0[d] - 1 + r % TS::w(d);
+                r /= TS::w(d);
                 ok = ok && k >= 0 && k < V.n[c][d];
                 node += k * stride;
                 stride *= V.n[c][d];
@@ -263,7 +269,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
     for (int c = 0; c < D; ++c)
 #pragma unroll
         for (int it = 0; it < NI; ++it)
-            if (tid + it * kBlock < win_size<D>()) sV[c][tid + it * kBlock] = wv[c][it];
+            if (tid + it * kBlock < TS::size()) sV[c][tid + it * kBlock] = wv[c][it];
     __syncthreads();
     int I[D];
     bool in = true;
@@ -271,8 +277,8 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
         int r = tid;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            I[d] = t0[d] + r % tile_n<D>(d);
-            r /= tile_n<D>(d);
+            I[d] = t0[d] + r % TS::n(d);
+            r /= TS::n(d);
             in = in && I[d] < g.ni[d];
         }
     }
@@ -304,10 +310,47 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
                 for (int q = 1; q < B; ++q) v = j == q ? ub[q][d] : v;
                 u0[d] = v;
             }
-            advect_split_one<D, M>(sV, V, g, sc, t0, I, P, u0, out);
+            advect_split_one<D, M, TS>(sV, V, g, sc, t0, I, P, u0, out);
 #pragma unroll
             for (int d = 0; d < D; ++d) off.p[d][c + C * (k0 + j)] = out[d];
         }
+    }
+}
+
+template <int D, int M>
+__global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, double dt,
+                                                      FPtr3 off, const uint8_t* __restrict__ index,
+                                                      RKf P) {
+    using TS = DefaultTile<D>;
+    __shared__ float sV[D][TS::size()];
+    adv_split_tile<D, M, TS>(g, V, dt, off, index, P, blockIdx.x, sV);
+}
+
+// ------------------------------------------------------------ ensemble
+// Many independent 2D grids advected in one persistent launch: block b walks
+// the global tile list t = b, b + gridDim.x, ...; tile t belongs to grid
+// tile_grid[t] and is that grid's tile t - tile0[grid].
+struct EnsGrid {
+    SGrid<2> g;
+    SVelIn<2> V;
+    double dt;
+    FPtr3 off;
+    const uint8_t* index;
+};
+
+template <int M, class TS>
+__global__ void __launch_bounds__(kBlock) k_adv_split_ens(const EnsGrid* __restrict__ grids,
+                                                          const uint16_t* __restrict__ tile_grid,
+                                                          const int32_t* __restrict__ tile0,
+                                                          int total, RKf P, int zero_dt) {
+    __shared__ float sV[2][TS::size()];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = tile_grid[t];
+        const EnsGrid& E = grids[i];
+        // zero_dt: the autotuner's timing runs (same work, positions unchanged)
+        adv_split_tile<2, M, TS>(E.g, E.V, zero_dt ? 0.0 : E.dt, E.off, E.index, P, t - tile0[i],
+                                 sV);
+        __syncthreads();  // the window is refilled for the next tile
     }
 }
 
@@ -903,7 +946,8 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     P.w0 = (float)std::fma(-0.5, ia, 1.0);
     P.w1 = (float)(0.5 * ia);
     int64_t tiles = 1;
-    for (int d = 0; d < D; ++d) tiles *= (G.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
+    for (int d = 0; d < D; ++d)
+        tiles *= (G.ni[d] + DefaultTile<D>::n(d) - 1) / DefaultTile<D>::n(d);
     FPtr3 fp = fptr3<D>(off);
     if (method == JP_EULER)
         k_adv_split<D, JP_EULER><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
@@ -980,6 +1024,77 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     }
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);
+}
+
+// ------------------------------------------------------------ ensemble host
+// Variants: tile shape (64x4, 32x8, 16x16, 128x2) x persistent blocks per CU (2, 8).
+constexpr int kEnsShapes = 4;
+constexpr int kEnsVariants = 8;
+
+int ens_tx(int s) { return s == 0 ? 64 : (s == 1 ? 32 : (s == 2 ? 16 : 128)); }
+int ens_ty(int s) { return kBlock / ens_tx(s); }
+
+template <int M>
+void ens_launch(hipStream_t st, int shape, unsigned blocks, const EnsGrid* grids,
+                const uint16_t* tg, const int32_t* t0, int total, const RKf& P, int zero_dt) {
+    switch (shape) {
+    case 0:
+        k_adv_split_ens<M, Tile<2, 64, 4>><<<blocks, kBlock, 0, st>>>(grids, tg, t0, total, P, zero_dt);
+        break;
+    case 1:
+        k_adv_split_ens<M, Tile<2, 32, 8>><<<blocks, kBlock, 0, st>>>(grids, tg, t0, total, P, zero_dt);
+        break;
+    case 2:
+        k_adv_split_ens<M, Tile<2, 16, 16>><<<blocks, kBlock, 0, st>>>(grids, tg, t0, total, P, zero_dt);
+        break;
+    default:
+        k_adv_split_ens<M, Tile<2, 128, 2>><<<blocks, kBlock, 0, st>>>(grids, tg, t0, total, P, zero_dt);
+        break;
+    }
+}
+
+}  // namespace
+
+struct jp_ensemble {
+    int n = 0;
+    int method = JP_RK2;
+    RKf P{};
+    EnsGrid* grids = nullptr;             // device
+    uint16_t* tile_grid[kEnsShapes] = {};  // device, per tile shape
+    int32_t* tile0[kEnsShapes] = {};       // device, per tile shape
+    int total[kEnsShapes] = {};
+    int variant = 1;  // 64x4 tiles, 8 blocks per CU (variant = 2*shape + (8 blocks per CU))
+};
+
+namespace {
+
+int ens_run(jp_ctx* ctx, jp_ensemble* e, int variant, int zero_dt) {
+    const int shape = variant >> 1;
+    const int bpc = (variant & 1) ? 8 : 2;
+    const int total = e->total[shape];
+    if (total == 0) return JP_OK;
+    const unsigned blocks = (unsigned)std::min<int64_t>(total, (int64_t)ctx->num_cu * bpc);
+    if (e->method == JP_EULER)
+        ens_launch<JP_EULER>(ctx->stream, shape, blocks, e->grids, e->tile_grid[shape],
+                             e->tile0[shape], total, e->P, zero_dt);
+    else if (e->method == JP_RK2)
+        ens_launch<JP_RK2>(ctx->stream, shape, blocks, e->grids, e->tile_grid[shape],
+                           e->tile0[shape], total, e->P, zero_dt);
+    else
+        ens_launch<JP_RK4>(ctx->stream, shape, blocks, e->grids, e->tile_grid[shape],
+                           e->tile0[shape], total, e->P, zero_dt);
+    JP_LAUNCHED(ctx);
+    return JP_OK;
+}
+
+void ens_free(jp_ensemble* e) {
+    if (!e) return;
+    (void)hipFree(e->grids);
+    for (int s = 0; s < kEnsShapes; ++s) {
+        (void)hipFree(e->tile_grid[s]);
+        (void)hipFree(e->tile0[s]);
+    }
+    delete e;
 }
 
 }  // namespace
@@ -1074,4 +1189,142 @@ int jp_inject_particles_split(jp_ctx* ctx, const jp_cellgrid* g, float* const* o
                : run_inject<3>(ctx, g, off, index, nargs, args, min_xcell, seed, call_counter, stats);
 }
 
+
+int jp_ensemble_create(jp_ctx* ctx, int n, const jp_cellgrid* grids, const jp_velgrid* vgs,
+                       float* const* off, uint8_t* const* index, const double* const* V,
+                       const double* dt, int method, double alpha, jp_ensemble** out) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!out || !grids || !vgs || !off || !index || !V || !dt)
+        return fail(ctx, JP_ERR_ARG, "null ensemble argument%s");
+    *out = nullptr;
+    if (n < 1 || n > 65535) return fail(ctx, JP_ERR_ARG, "ensemble size%s %lld out of range", "", n);
+    if (method < JP_EULER || method > JP_RK4)
+        return fail(ctx, JP_ERR_ARG, "unknown integrator%s %lld", "", method);
+    if (method == JP_RK2 && !(0.0 < alpha && alpha < 1.0))
+        return fail(ctx, JP_ERR_ARG, "Only 0 < α < 1 is supported%s");
+    if (ctx->capturing) return fail(ctx, JP_ERR_STATE, "ensemble created during capture%s");
+    std::vector<EnsGrid> host(n);
+    for (int i = 0; i < n; ++i) {
+        const jp_cellgrid* g = grids + i;
+        if (g->ndim != 2) return fail(ctx, JP_ERR_ARG, "ensemble grid%s %lld is not 2D", "", i);
+        float* const o2[2] = {off[2 * i], off[2 * i + 1]};
+        int rc = check_common(ctx, g, (const void* const*)o2, index[i]);
+        if (rc) return rc;
+        if (!V[2 * i] || !V[2 * i + 1]) return fail(ctx, JP_ERR_ARG, "null velocity of grid%s %lld", "", i);
+        Grid<2> G = to_grid<2>(g);
+        EnsGrid& E = host[i];
+        E.g = sgrid<2>(G);
+        for (int c = 0; c < 2; ++c) {
+            E.V.v[c] = V[2 * i + c];
+            for (int d = 0; d < 2; ++d) {
+                E.V.n[c][d] = (int)vgs[i].ax[c][d].n;
+                const int64_t want = d == c ? G.ni[d] + 1 : G.ni[d] + 2;
+                if (vgs[i].ax[c][d].n != want)
+                    return fail(ctx, JP_ERR_ARG, "ensemble grid%s %lld: non-standard staggering", "", i);
+            }
+        }
+        E.dt = dt[i];
+        E.off = FPtr3{{o2[0], o2[1], nullptr}};
+        E.index = index[i];
+    }
+    jp_ensemble* e = new jp_ensemble();
+    e->n = n;
+    e->method = method;
+    e->P.alpha = (float)alpha;
+    e->P.half = alpha == 0.5;
+    const double ia = 1.0 / alpha;
+    e->P.w0 = (float)std::fma(-0.5, ia, 1.0);
+    e->P.w1 = (float)(0.5 * ia);
+    hipError_t err = hipMalloc(&e->grids, sizeof(EnsGrid) * n);
+    if (err == hipSuccess)
+        err = hipMemcpy(e->grids, host.data(), sizeof(EnsGrid) * n, hipMemcpyHostToDevice);
+    for (int s = 0; s < kEnsShapes && err == hipSuccess; ++s) {
+        std::vector<int32_t> t0(n);
+        std::vector<uint16_t> tg;
+        int64_t tot = 0;
+        for (int i = 0; i < n; ++i) {
+            t0[i] = (int32_t)tot;
+            const int64_t tiles = ((host[i].g.ni[0] + ens_tx(s) - 1) / ens_tx(s)) *
+                                  (int64_t)((host[i].g.ni[1] + ens_ty(s) - 1) / ens_ty(s));
+            for (int64_t t = 0; t < tiles; ++t) tg.push_back((uint16_t)i);
+            tot += tiles;
+        }
+        if (tot >= ((int64_t)1 << 31)) {
+            ens_free(e);
+            return fail(ctx, JP_ERR_ARG, "ensemble too large%s");
+        }
+        e->total[s] = (int)tot;
+        err = hipMalloc(&e->tile_grid[s], sizeof(uint16_t) * std::max<int64_t>(tot, 1));
+        if (err == hipSuccess) err = hipMalloc(&e->tile0[s], sizeof(int32_t) * n);
+        if (err == hipSuccess && tot)
+            err = hipMemcpy(e->tile_grid[s], tg.data(), sizeof(uint16_t) * tot, hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(e->tile0[s], t0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    }
+    if (err != hipSuccess) {
+        ens_free(e);
+        return fail(ctx, JP_ERR_HIP, "ensemble allocation: %s", hipGetErrorString(err));
+    }
+    *out = e;
+    return JP_OK;
+}
+
+int jp_ensemble_advect(jp_ctx* ctx, jp_ensemble* e) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!e) return fail(ctx, JP_ERR_ARG, "null ensemble%s");
+    int rc = ens_run(ctx, e, e->variant, 0);
+    if (rc) return rc;
+    return finish(ctx);
+}
+
+int jp_ensemble_set_variant(jp_ctx* ctx, jp_ensemble* e, int variant) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!e) return fail(ctx, JP_ERR_ARG, "null ensemble%s");
+    if (variant < 0 || variant >= kEnsVariants)
+        return fail(ctx, JP_ERR_ARG, "variant%s %lld out of range", "", variant);
+    e->variant = variant;
+    return JP_OK;
+}
+
+int jp_ensemble_tune(jp_ctx* ctx, jp_ensemble* e, int reps, int* best, float* ms) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!e || reps < 1) return fail(ctx, JP_ERR_ARG, "bad ensemble tuning arguments%s");
+    if (ctx->capturing) return fail(ctx, JP_ERR_STATE, "tuning during capture%s");
+    hipEvent_t a, b;
+    JP_HIP(ctx, hipEventCreate(&a));
+    JP_HIP(ctx, hipEventCreate(&b));
+    int bv = 0;
+    float bt = 0.0f;
+    int rc = JP_OK;
+    for (int v = 0; v < kEnsVariants && !rc; ++v) {
+        // one untimed launch, then reps timed ones; dt = 0 leaves positions unchanged
+        rc = ens_run(ctx, e, v, 1);
+        if (!rc && hipEventRecord(a, ctx->stream) != hipSuccess) rc = fail(ctx, JP_ERR_HIP, "event%s");
+        for (int r = 0; r < reps && !rc; ++r) rc = ens_run(ctx, e, v, 1);
+        if (!rc && hipEventRecord(b, ctx->stream) != hipSuccess) rc = fail(ctx, JP_ERR_HIP, "event%s");
+        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = fail(ctx, JP_ERR_HIP, "event sync%s");
+        float t = 0.0f;
+        if (!rc && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = fail(ctx, JP_ERR_HIP, "elapsed%s");
+        t /= (float)reps;
+        if (ms) ms[v] = t;
+        if (!rc && (v == 0 || t < bt)) {
+            bt = t;
+            bv = v;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    e->variant = bv;
+    if (best) *best = bv;
+    return JP_OK;
+}
+
+int jp_ensemble_destroy(jp_ctx* ctx, jp_ensemble* e) {
+    if (!ctx) return JP_ERR_STATE;
+    if (ctx->capturing) return fail(ctx, JP_ERR_STATE, "ensemble destroyed during capture%s");
+    (void)hipStreamSynchronize(ctx->stream);
+    ens_free(e);
+    return JP_OK;
+}
 }  // extern "C"

@@ -13,12 +13,14 @@ from .api import (AbstractAdvectionIntegrator, Euler, RungeKutta2, RungeKutta4, 
                   grid2particle_, inject_particles_, move_particles_, nodal_field,
                   particle2grid_)
 from .device import Context, DeviceArray, JustPICError
+from .ensemble import Ensemble, random_ensemble
 from .grids import Axis, expand_range, linrange, staggered_grid, steprange
 from .particles import (CellArray, Particles, SplitParticles, init_cell_arrays,
                         init_particles, particles_from_host, to_f64, to_split)
 
 __all__ = [
-    "AbstractAdvectionIntegrator", "Axis", "CellArray", "Context", "DeviceArray", "Euler",
+    "AbstractAdvectionIntegrator", "Axis", "CellArray", "Context", "DeviceArray", "Ensemble",
+    "Euler", "random_ensemble",
     "HipLibraryMissing", "JustPICError", "Particles", "RungeKutta2", "RungeKutta4",
     "advection_", "expand_range", "grid2particle_", "init_cell_arrays", "init_particles",
     "inject_particles_", "linrange", "load_library", "move_particles_", "nodal_field",

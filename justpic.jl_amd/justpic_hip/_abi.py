@@ -16,6 +16,7 @@ JP_OK = 0
 JP_ERR_ARG, JP_ERR_HIP, JP_ERR_STATE = -1, -2, -3
 JP_EULER, JP_RK2, JP_RK4 = 0, 1, 2
 JP_MAX_ARGS = 16
+JP_ENS_VARIANTS = 8
 
 LIB_NAME = "libjustpic_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -118,6 +119,12 @@ _SIGS = {
     "jp_inject_particles_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
                                             C.c_int, C.c_uint64, C.c_uint64,
                                             C.POINTER(jp_stats)]),
+    "jp_ensemble_create": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int, C.c_double,
+                                     C.POINTER(C.c_void_p)]),
+    "jp_ensemble_advect": (C.c_int, [_P, _P]),
+    "jp_ensemble_set_variant": (C.c_int, [_P, _P, C.c_int]),
+    "jp_ensemble_tune": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), _P]),
+    "jp_ensemble_destroy": (C.c_int, [_P, _P]),
 }
 
 
