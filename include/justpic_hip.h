@@ -217,6 +217,43 @@ int jp_ensemble_set_variant(jp_ctx* ctx, jp_ensemble* ens, int variant);
 int jp_ensemble_tune(jp_ctx* ctx, jp_ensemble* ens, int reps, int* best, float* ms);
 int jp_ensemble_destroy(jp_ctx* ctx, jp_ensemble* ens);
 
+/* ---- adjoint of advection + particle2grid (no reference counterpart) ----
+ * A flat list of np particles (positions x0[d][i], tracer Tp[i], FP64,
+ * device memory) is advected nsteps times with method/alpha/dt exactly as
+ * advection! advects one particle (src/Particles/Advection/advection.jl:
+ * 66-91); a particle whose position leaves the open domain is deleted, as
+ * move_particles! deletes it (src/Particles/move_safe.jl:99-103); the
+ * survivors are then gathered onto the nodal grid as particle2grid! does
+ * (src/Interpolations/particle_to_grid.jl:37-68, 113-151): F = sum(w T) /
+ * sum(w), w = inv(distance^2), over the particles of the 2^D cells around a
+ * node.  The list has no slot capacity, so it does not model particles that
+ * move_particles! drops when a destination cell is full.
+ *
+ * jp_adjoint_forward writes the positions at steps 0, every, 2*every, ...
+ * (ckpt, max(1, ceil(nsteps/every)) * ndim * np doubles, at
+ * [(s/every)*ndim + d]*np + i; block 0 holds x0 also when nsteps == 0),
+ * alive[i] (0 once deleted), the final positions xN, F and the weight sums W
+ * (nodal, (ni+1) column-major; a node without particles gets NaN).
+ * jp_adjoint_backward takes the cotangent Fbar of F and returns the
+ * cotangents of V (Vbar[c], overwritten), x0 and Tp (0 for deleted
+ * particles), recomputing every block of `every` steps from its checkpoint.
+ * The derivative is taken on the branch the forward pass takes (brackets and
+ * cells fixed), with w = 1/|x - x_node|^2.  Vbar is accumulated with atomic
+ * adds, so its rounding depends on their order; x0bar and Tpbar do not.
+ * Methods: JP_EULER, JP_RK2, JP_RK4; 1 <= every <= JP_ADJ_MAX_EVERY. */
+#define JP_ADJ_MAX_EVERY 16
+int jp_adjoint_forward(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
+                       const double* const* V, int method, double alpha, double dt,
+                       int nsteps, int every, int64_t np, const double* const* x0,
+                       const double* Tp, double* ckpt, uint8_t* alive, double* const* xN,
+                       double* F, double* W);
+int jp_adjoint_backward(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
+                        const double* const* V, int method, double alpha, double dt,
+                        int nsteps, int every, int64_t np, const double* ckpt,
+                        const uint8_t* alive, const double* Tp, const double* F,
+                        const double* W, const double* Fbar, double* const* Vbar,
+                        double* const* x0bar, double* Tpbar);
+
 /* Philox4x32-10 block (host helper, for tests): out[4] <- philox(ctr[4], key[2]) */
 void jp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

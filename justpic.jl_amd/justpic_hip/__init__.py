@@ -5,10 +5,13 @@ Reference names map as ``f!`` -> ``f_``: ``advection_``, ``move_particles_``,
 ``particle2grid_``, ``grid2particle_``, ``inject_particles_``; containers are
 ``Particles``/``CellArray``; integrators ``Euler``, ``RungeKutta2``,
 ``RungeKutta4``.  ``init_particles(..., form="split")`` / ``to_split`` give the
-FP32-split position form, accepted by the same functions.  Importing this package does not load the library; the first
-``Context`` does, and raises if it is not built.
+FP32-split position form, accepted by the same functions.  ``Ensemble`` runs
+many independent grids in one launch; ``rrule_advection_p2g`` is the reverse
+mode of advection + particle2grid.  Importing this package does not load the
+library; the first ``Context`` does, and raises if it is not built.
 """
 from ._abi import HipLibraryMissing, load_library
+from .adjoint import AdvectionP2G, flatten, l2_loss, rrule_advection_p2g
 from .api import (AbstractAdvectionIntegrator, Euler, RungeKutta2, RungeKutta4, advection_,
                   grid2particle_, inject_particles_, move_particles_, nodal_field,
                   particle2grid_)
@@ -19,7 +22,8 @@ from .particles import (CellArray, Particles, SplitParticles, init_cell_arrays,
                         init_particles, particles_from_host, to_f64, to_split)
 
 __all__ = [
-    "AbstractAdvectionIntegrator", "Axis", "CellArray", "Context", "DeviceArray", "Ensemble",
+    "AbstractAdvectionIntegrator", "AdvectionP2G", "flatten", "l2_loss", "rrule_advection_p2g",
+    "Axis", "CellArray", "Context", "DeviceArray", "Ensemble",
     "Euler", "random_ensemble",
     "HipLibraryMissing", "JustPICError", "Particles", "RungeKutta2", "RungeKutta4",
     "advection_", "expand_range", "grid2particle_", "init_cell_arrays", "init_particles",

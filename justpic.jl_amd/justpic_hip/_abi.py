@@ -17,6 +17,7 @@ JP_ERR_ARG, JP_ERR_HIP, JP_ERR_STATE = -1, -2, -3
 JP_EULER, JP_RK2, JP_RK4 = 0, 1, 2
 JP_MAX_ARGS = 16
 JP_ENS_VARIANTS = 8
+JP_ADJ_MAX_EVERY = 16
 
 LIB_NAME = "libjustpic_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -125,6 +126,11 @@ _SIGS = {
     "jp_ensemble_set_variant": (C.c_int, [_P, _P, C.c_int]),
     "jp_ensemble_tune": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), _P]),
     "jp_ensemble_destroy": (C.c_int, [_P, _P]),
+    "jp_adjoint_forward": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, C.c_int64, _P, _P, _P, _P, _P, _P, _P]),
+    "jp_adjoint_backward": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P,
+                                      _P]),
 }
 
 

@@ -761,3 +761,321 @@ int jpo_inject_particles(const jp_cellgrid* g, double* const* coords,
     }
     return JP_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Adjoint of advection + particle2grid on a flat particle list        */
+/* ------------------------------------------------------------------ */
+
+/* No reference counterpart: JustPIC.jl has no adjoint.  The forward model
+ * being differentiated follows the reference for every particle that is not
+ * lost to a full destination cell:
+ *  - one step is advect_particle above (src/Particles/Advection/advection.jl:
+ *    66-91 with Euler.jl, RK2.jl, RK4.jl), every bracket seeded at the
+ *    uniform-spacing estimate (the seed changes nothing for a point strictly
+ *    inside one interval; on a tie both this and the HIP library run the
+ *    reference bisection from that seed);
+ *  - a particle whose new position is not strictly inside the domain is
+ *    deleted, as move_particles! deletes it (src/Particles/move_safe.jl:99-103);
+ *  - after the last step, particle2grid! (src/Interpolations/
+ *    particle_to_grid.jl:37-68, 113-151): every node accumulates w =
+ *    inv(distance^2) and w*T over the particles of its 2^D adjacent cells,
+ *    F = wT / w in 2D and wT * inv(w) in 3D, accumulated here in particle
+ *    order.  A particle's cell is the bracket of its position on the vertex
+ *    axes.
+ * The adjoint is the exact derivative of this model on the branch the forward
+ * pass takes (brackets and cells held fixed), with w taken as 1/s,
+ * s = |x - x_node|^2.  It recomputes each trajectory in full (no
+ * checkpoints), which makes it independent of the HIP library's checkpoint
+ * schedule. */
+
+/* bracket (0-based interval) of p in [x[0], x[n-1]], bisection seeded at the
+ * uniform-spacing estimate */
+static int64_t adj_bracket(const jp_axis* a, double p) {
+    int64_t last = a->n - 2;
+    int64_t s = (int64_t)((p - a->x[0]) * a->inv_dx[0]);
+    if (s < 0) s = 0;
+    if (s > last) s = last;
+    return jpo_bisect(p, a->x, a->n, s + 1) - 1;
+}
+
+/* velocity component c at p (+Inf outside the component's grid, as
+ * interp_velocity above); with Vb non-NULL also its adjoint for the cotangent
+ * vb: Vb[c][corner] += weight*vb and pb[d] += dv/dp_d * vb. */
+static double adj_interp(const jp_velgrid* vg, const double* const* V, int D, int c,
+                         const double* p, double vb, double* const* Vb, double* pb) {
+    const jp_axis* ax = vg->ax[c];
+    if (!check_limits(ax, D, p)) return INFINITY;
+    int64_t k[NDIM_MAX] = {0, 0, 0};
+    double t[NDIM_MAX] = {0, 0, 0}, iv[NDIM_MAX] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        k[d] = adj_bracket(&ax[d], p[d]);
+        iv[d] = 1.0 / ax[d].dx[k[d]];
+        t[d] = (p[d] - ax[d].x[k[d]]) * iv[d];
+    }
+    int64_t n0 = ax[0].n, n1 = ax[1].n;
+    int nc = (D == 2) ? 4 : 8;
+    int64_t at[8];
+    double v[8];
+    for (int q = 0; q < nc; ++q) {
+        int64_t a = k[0] + (q & 1), b = k[1] + ((q >> 1) & 1);
+        int64_t e = (D == 3) ? k[2] + ((q >> 2) & 1) : 0;
+        at[q] = a + n0 * (b + n1 * e);
+        v[q] = V[c][at[q]];
+    }
+    double val = (D == 2) ? jpo_lerp2(v, t) : jpo_lerp3(v, t);
+    if (!Vb) return val;
+    /* n-linear interpolation: weight of corner q is prod_d (bit ? t : 1-t) */
+    double dd[NDIM_MAX] = {0, 0, 0};
+    if (D == 2) {
+        dd[0] = (1.0 - t[1]) * (v[1] - v[0]) + t[1] * (v[3] - v[2]);
+        dd[1] = (1.0 - t[0]) * (v[2] - v[0]) + t[0] * (v[3] - v[1]);
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            double s = 0.0;
+            for (int q = 0; q < 8; ++q) {
+                if ((q >> d) & 1) continue;
+                int q1 = q | (1 << d);
+                double w = 1.0;
+                for (int e = 0; e < 3; ++e) {
+                    if (e == d) continue;
+                    w *= ((q >> e) & 1) ? t[e] : 1.0 - t[e];
+                }
+                s += w * (v[q1] - v[q]);
+            }
+            dd[d] = s;
+        }
+    }
+    for (int d = 0; d < D; ++d) pb[d] += dd[d] * iv[d] * vb;
+    for (int q = 0; q < nc; ++q) {
+        double w = 1.0;
+        for (int d = 0; d < D; ++d) w *= ((q >> d) & 1) ? t[d] : 1.0 - t[d];
+        Vb[c][at[q]] += w * vb;
+    }
+    return val;
+}
+
+static void adj_velocity(const jp_velgrid* vg, const double* const* V, int D,
+                         const double* p, double* v) {
+    for (int c = 0; c < D; ++c) v[c] = adj_interp(vg, V, D, c, p, 0.0, NULL, NULL);
+}
+
+/* one step: the arithmetic of advect_particle with adjoint-model brackets */
+static void adj_step(int method, double alpha, double dt, int D, const jp_velgrid* vg,
+                     const double* const* V, const double* p0, double* out) {
+    double v0[NDIM_MAX] = {0, 0, 0}, v1[NDIM_MAX] = {0, 0, 0}, p1[NDIM_MAX] = {0, 0, 0};
+    if (method == JP_EULER) {
+        adj_velocity(vg, V, D, p0, v0);
+        jpo_first_stage(JP_EULER, alpha, dt, D, v0, p0, out);
+    } else if (method == JP_RK2) {
+        adj_velocity(vg, V, D, p0, v0);
+        jpo_first_stage(JP_RK2, alpha, dt, D, v0, p0, p1);
+        adj_velocity(vg, V, D, p1, v1);
+        jpo_second_stage(alpha, dt, D, v0, v1, p0, out);
+    } else {
+        double k1[NDIM_MAX], k2[NDIM_MAX], k3[NDIM_MAX], k4[NDIM_MAX], q[NDIM_MAX];
+        adj_velocity(vg, V, D, p0, k1);
+        for (int d = 0; d < D; ++d) q[d] = p0[d] + (dt * k1[d]) / 2.0;
+        adj_velocity(vg, V, D, q, k2);
+        for (int d = 0; d < D; ++d) q[d] = p0[d] + (dt * k2[d]) / 2.0;
+        adj_velocity(vg, V, D, q, k3);
+        for (int d = 0; d < D; ++d) q[d] = p0[d] + dt * k3[d];
+        adj_velocity(vg, V, D, q, k4);
+        for (int d = 0; d < D; ++d) {
+            double sum = ((k1[d] + 2.0 * k2[d]) + 2.0 * k3[d]) + k4[d];
+            out[d] = p0[d] + (dt * sum) / 6.0;
+        }
+    }
+}
+
+/* reverse of adj_step at p0: pb holds the cotangent of the step's output on
+ * entry and that of p0 on return; velocity cotangents accumulate into Vb */
+static void adj_step_bw(int method, double alpha, double dt, int D, const jp_velgrid* vg,
+                        const double* const* V, const double* p0, double* pb,
+                        double* const* Vb) {
+    double ob[NDIM_MAX];
+    for (int d = 0; d < D; ++d) ob[d] = pb[d];
+    if (method == JP_EULER) {
+        for (int c = 0; c < D; ++c) adj_interp(vg, V, D, c, p0, dt * ob[c], Vb, pb);
+    } else if (method == JP_RK2) {
+        double v0[NDIM_MAX], p1[NDIM_MAX], v0b[NDIM_MAX], v1b[NDIM_MAX];
+        double p1b[NDIM_MAX] = {0, 0, 0};
+        adj_velocity(vg, V, D, p0, v0);
+        jpo_first_stage(JP_RK2, alpha, dt, D, v0, p0, p1);
+        double adt = (1.0 * alpha) * dt;
+        if (alpha == 0.5) {
+            for (int d = 0; d < D; ++d) {
+                v1b[d] = dt * ob[d];
+                v0b[d] = 0.0;
+            }
+        } else {
+            double ia = 1.0 / alpha, w0 = fma(-0.5, ia, 1.0), w1 = 0.5 * ia;
+            for (int d = 0; d < D; ++d) {
+                v1b[d] = dt * w1 * ob[d];
+                v0b[d] = dt * w0 * ob[d];
+            }
+        }
+        for (int c = 0; c < D; ++c) adj_interp(vg, V, D, c, p1, v1b[c], Vb, p1b);
+        for (int d = 0; d < D; ++d) {
+            pb[d] += p1b[d];
+            v0b[d] += adt * p1b[d];
+        }
+        for (int c = 0; c < D; ++c) adj_interp(vg, V, D, c, p0, v0b[c], Vb, pb);
+    } else {
+        double k1[NDIM_MAX], k2[NDIM_MAX], k3[NDIM_MAX], q2[NDIM_MAX], q3[NDIM_MAX],
+            q4[NDIM_MAX];
+        double kb[4][NDIM_MAX], qb[NDIM_MAX];
+        adj_velocity(vg, V, D, p0, k1);
+        for (int d = 0; d < D; ++d) q2[d] = p0[d] + (dt * k1[d]) / 2.0;
+        adj_velocity(vg, V, D, q2, k2);
+        for (int d = 0; d < D; ++d) q3[d] = p0[d] + (dt * k2[d]) / 2.0;
+        adj_velocity(vg, V, D, q3, k3);
+        for (int d = 0; d < D; ++d) q4[d] = p0[d] + dt * k3[d];
+        for (int d = 0; d < D; ++d) {
+            kb[0][d] = dt / 6.0 * ob[d];
+            kb[1][d] = 2.0 * dt / 6.0 * ob[d];
+            kb[2][d] = 2.0 * dt / 6.0 * ob[d];
+            kb[3][d] = dt / 6.0 * ob[d];
+        }
+        const double* qs[3] = {q2, q3, q4};
+        const double fac[3] = {dt / 2.0, dt / 2.0, dt};
+        for (int st = 3; st >= 1; --st) { /* k_{st+1} = v(q_{st+1}), q = p0 + fac*k_st */
+            for (int d = 0; d < D; ++d) qb[d] = 0.0;
+            for (int c = 0; c < D; ++c) adj_interp(vg, V, D, c, qs[st - 1], kb[st][c], Vb, qb);
+            for (int d = 0; d < D; ++d) {
+                pb[d] += qb[d];
+                kb[st - 1][d] += fac[st - 1] * qb[d];
+            }
+        }
+        for (int c = 0; c < D; ++c) adj_interp(vg, V, D, c, p0, kb[0][c], Vb, pb);
+    }
+}
+
+/* strictly inside the domain (move_safe.jl:199-206); NaN and Inf fail */
+static int adj_indomain(const jp_cellgrid* g, const double* p) {
+    for (int d = 0; d < g->ndim; ++d) {
+        const jp_axis* a = &g->xv[d];
+        if (!(a->x[0] < p[d] && p[d] < a->x[a->n - 1])) return 0;
+    }
+    return 1;
+}
+
+static int adj_check(const jp_cellgrid* g, int method, double alpha, int nsteps, int every,
+                     int64_t np) {
+    if (check_grid(g)) return JP_ERR_ARG;
+    if (method < JP_EULER || method > JP_RK4) return JP_ERR_ARG;
+    if (method == JP_RK2 && !(0.0 < alpha && alpha < 1.0)) return JP_ERR_ARG;
+    if (nsteps < 0 || np < 0) return JP_ERR_ARG;
+    if (every < 1 || every > JP_ADJ_MAX_EVERY) return JP_ERR_ARG;
+    return JP_OK;
+}
+
+static int64_t adj_nodes(const jp_cellgrid* g, int64_t* nn) {
+    int64_t NN = 1;
+    for (int d = 0; d < NDIM_MAX; ++d) nn[d] = 1;
+    for (int d = 0; d < g->ndim; ++d) {
+        nn[d] = g->ni[d] + 1;
+        NN *= nn[d];
+    }
+    return NN;
+}
+
+/* node q (bit d: +1 along d) of the cell holding p: linear node index and
+ * weight inv(distance^2) */
+static int64_t adj_node(const jp_cellgrid* g, const int64_t* nn, const int64_t* k, int q,
+                        const double* p, double* xn, double* wi) {
+    int D = g->ndim;
+    int64_t N[NDIM_MAX] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        N[d] = k[d] + ((q >> d) & 1);
+        xn[d] = g->xv[d].x[N[d]];
+    }
+    double dist = jpo_distance(D, xn, p);
+    *wi = 1.0 / (dist * dist);
+    return N[0] + nn[0] * (N[1] + nn[1] * N[2]);
+}
+
+int jpo_adjoint_forward(const jp_cellgrid* g, const jp_velgrid* vg, const double* const* V,
+                        int method, double alpha, double dt, int nsteps, int every,
+                        int64_t np, const double* const* x0, const double* Tp,
+                        double* ckpt, uint8_t* alive, double* const* xN, double* F,
+                        double* W) {
+    if (adj_check(g, method, alpha, nsteps, every, np)) return JP_ERR_ARG;
+    int D = g->ndim;
+    int64_t nn[NDIM_MAX];
+    int64_t NN = adj_nodes(g, nn);
+    for (int64_t n = 0; n < NN; ++n) F[n] = W[n] = 0.0;
+    for (int64_t i = 0; i < np; ++i) {
+        double x[NDIM_MAX], y[NDIM_MAX];
+        for (int d = 0; d < D; ++d) x[d] = x0[d][i];
+        int a = adj_indomain(g, x);
+        if (ckpt) /* block 0 always exists, also for nsteps == 0 */
+            for (int d = 0; d < D; ++d) ckpt[(int64_t)d * np + i] = x[d];
+        for (int s = 0; s < nsteps; ++s) {
+            if (ckpt && s > 0 && s % every == 0)
+                for (int d = 0; d < D; ++d) ckpt[((int64_t)(s / every) * D + d) * np + i] = x[d];
+            if (!a) continue;
+            adj_step(method, alpha, dt, D, vg, V, x, y);
+            for (int d = 0; d < D; ++d) x[d] = y[d];
+            a = adj_indomain(g, x);
+        }
+        alive[i] = (uint8_t)a;
+        for (int d = 0; d < D; ++d) xN[d][i] = x[d];
+        if (!a) continue;
+        int64_t k[NDIM_MAX] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) k[d] = adj_bracket(&g->xv[d], x[d]);
+        for (int q = 0; q < (1 << D); ++q) {
+            double xn[NDIM_MAX], wi;
+            int64_t n = adj_node(g, nn, k, q, x, xn, &wi);
+            W[n] += wi;
+            F[n] = fma(wi, Tp[i], F[n]);
+        }
+    }
+    for (int64_t n = 0; n < NN; ++n) F[n] = (D == 2) ? F[n] / W[n] : F[n] * (1.0 / W[n]);
+    return JP_OK;
+}
+
+int jpo_adjoint_backward(const jp_cellgrid* g, const jp_velgrid* vg, const double* const* V,
+                         int method, double alpha, double dt, int nsteps, int64_t np,
+                         const double* const* x0, const double* Tp, const double* F,
+                         const double* W, const double* Fbar, double* const* Vbar,
+                         double* const* x0bar, double* Tpbar) {
+    if (adj_check(g, method, alpha, nsteps, 1, np)) return JP_ERR_ARG;
+    int D = g->ndim;
+    int64_t nn[NDIM_MAX];
+    adj_nodes(g, nn);
+    for (int c = 0; c < D; ++c) {
+        int64_t m = 1;
+        for (int d = 0; d < D; ++d) m *= vg->ax[c][d].n;
+        for (int64_t e = 0; e < m; ++e) Vbar[c][e] = 0.0;
+    }
+    double* traj = (double*)malloc(sizeof(double) * (size_t)(nsteps + 1) * (size_t)D);
+    if (!traj) return JP_ERR_ARG;
+    for (int64_t i = 0; i < np; ++i) {
+        for (int d = 0; d < D; ++d) traj[d] = x0[d][i];
+        int a = adj_indomain(g, traj);
+        for (int s = 0; a && s < nsteps; ++s) {
+            adj_step(method, alpha, dt, D, vg, V, traj + s * D, traj + (s + 1) * D);
+            a = adj_indomain(g, traj + (s + 1) * D);
+        }
+        double pb[NDIM_MAX] = {0, 0, 0}, tb = 0.0;
+        if (a) {
+            const double* x = traj + (int64_t)nsteps * D;
+            int64_t k[NDIM_MAX] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) k[d] = adj_bracket(&g->xv[d], x[d]);
+            for (int q = 0; q < (1 << D); ++q) {
+                double xn[NDIM_MAX], wi;
+                int64_t n = adj_node(g, nn, k, q, x, xn, &wi);
+                double iw = 1.0 / W[n];
+                tb += Fbar[n] * wi * iw;
+                double gw = Fbar[n] * (Tp[i] - F[n]) * iw;
+                for (int d = 0; d < D; ++d) pb[d] += gw * (-2.0 * (x[d] - xn[d]) * wi * wi);
+            }
+            for (int s = nsteps - 1; s >= 0; --s)
+                adj_step_bw(method, alpha, dt, D, vg, V, traj + s * D, pb, Vbar);
+        }
+        for (int d = 0; d < D; ++d) x0bar[d][i] = pb[d];
+        Tpbar[i] = tb;
+    }
+    free(traj);
+    return JP_OK;
+}

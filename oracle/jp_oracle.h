@@ -58,6 +58,23 @@ int jpo_inject_particles(const jp_cellgrid* g, double* const* coords,
                          int min_xcell, uint64_t seed, uint64_t call_counter,
                          jp_stats* stats);
 
+/* adjoint of advection + particle2grid on a flat particle list (the model of
+ * jp_adjoint_forward / jp_adjoint_backward in justpic_hip.h).  ckpt may be
+ * NULL.  The backward recomputes whole trajectories from x0 (no checkpoints).
+ * Pinned by central finite differences of jpo_adjoint_forward, by the
+ * uniform-velocity identity sum(Vbar_c) = nsteps*dt*sum(x0bar_c), and the
+ * forward by jpo_advection + jpo_move_particles + jpo_particle2grid. */
+int jpo_adjoint_forward(const jp_cellgrid* g, const jp_velgrid* vg, const double* const* V,
+                        int method, double alpha, double dt, int nsteps, int every,
+                        int64_t np, const double* const* x0, const double* Tp,
+                        double* ckpt, uint8_t* alive, double* const* xN, double* F,
+                        double* W);
+int jpo_adjoint_backward(const jp_cellgrid* g, const jp_velgrid* vg, const double* const* V,
+                         int method, double alpha, double dt, int nsteps, int64_t np,
+                         const double* const* x0, const double* Tp, const double* F,
+                         const double* W, const double* Fbar, double* const* Vbar,
+                         double* const* x0bar, double* Tpbar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,12 @@ def lib():
             "jpo_grid2particle": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, P]),
             "jpo_inject_particles": (C.c_int, [C.POINTER(jp_cellgrid), P, P, C.c_int, P, C.c_int,
                                                C.c_uint64, C.c_uint64, C.POINTER(jp_stats)]),
+            "jpo_adjoint_forward": (C.c_int, [C.POINTER(jp_cellgrid), C.POINTER(jp_velgrid), P,
+                                              C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                              C.c_int64, P, P, P, P, P, P, P]),
+            "jpo_adjoint_backward": (C.c_int, [C.POINTER(jp_cellgrid), C.POINTER(jp_velgrid), P,
+                                               C.c_int, C.c_double, C.c_double, C.c_int,
+                                               C.c_int64, P, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -226,7 +232,59 @@ def inject_particles(hp: HostParticles, args, min_xcell, seed, counter):
     return st.as_dict()
 
 
-__all__ = ["JP_EULER", "JP_RK2", "JP_RK4", "HostParticles", "init_particles", "advection",
+def _flat(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+def adjoint_forward(hp: HostParticles, V, method, alpha, dt, nsteps, x0, Tp, every=10):
+    """Flat-list model F = particle2grid(advect^nsteps(x0), Tp) (host arrays).
+    Returns dict(xN, alive, F, W, ckpt); F and W are flat nodal arrays."""
+    D = hp.D
+    V = [_flat(v) for v in V]
+    x0 = [_flat(x) for x in x0]
+    Tp = _flat(Tp)
+    n = Tp.size
+    nn = int(np.prod([k + 1 for k in hp.ni]))
+    nseg = max(1, -(-int(nsteps) // int(every)))
+    ckpt = np.zeros(max(1, nseg * D * n))
+    alive = np.zeros(max(1, n), dtype=np.uint8)
+    xN = [np.zeros(max(1, n)) for _ in range(D)]
+    F = np.zeros(nn)
+    W = np.zeros(nn)
+    vg = hp.velgrid()
+    _chk(lib().jpo_adjoint_forward(C.byref(hp._grid), C.byref(vg), ptr_array([_p(v) for v in V]),
+                                   int(method), float(alpha), float(dt), int(nsteps), int(every),
+                                   n, ptr_array([_p(x) for x in x0]), _p(Tp), _p(ckpt),
+                                   _p(alive), ptr_array([_p(x) for x in xN]), _p(F), _p(W)),
+         "adjoint forward")
+    return dict(xN=[x[:n] for x in xN], alive=alive[:n], F=F, W=W,
+                ckpt=ckpt[:nseg * D * n].reshape(nseg, D, n))
+
+
+def adjoint_backward(hp: HostParticles, V, method, alpha, dt, nsteps, x0, Tp, F, W, Fbar):
+    """Cotangents (Vbar tuple shaped like V, x0bar tuple, Tpbar) of the
+    flat-list model for the nodal cotangent Fbar."""
+    D = hp.D
+    shapes = [np.shape(v) for v in V]
+    V = [_flat(v) for v in V]
+    x0 = [_flat(x) for x in x0]
+    Tp, F, W, Fbar = (_flat(a) for a in (Tp, F, W, Fbar))
+    n = Tp.size
+    Vbar = [np.zeros_like(v) for v in V]
+    x0bar = [np.zeros(max(1, n)) for _ in range(D)]
+    Tpbar = np.zeros(max(1, n))
+    vg = hp.velgrid()
+    _chk(lib().jpo_adjoint_backward(C.byref(hp._grid), C.byref(vg), ptr_array([_p(v) for v in V]),
+                                    int(method), float(alpha), float(dt), int(nsteps), n,
+                                    ptr_array([_p(x) for x in x0]), _p(Tp), _p(F), _p(W),
+                                    _p(Fbar), ptr_array([_p(v) for v in Vbar]),
+                                    ptr_array([_p(x) for x in x0bar]), _p(Tpbar)),
+         "adjoint backward")
+    Vbar = tuple(v.reshape(s, order="F") for v, s in zip(Vbar, shapes))
+    return Vbar, [x[:n] for x in x0bar], Tpbar[:n]
+
+
+__all__ = ["adjoint_forward", "adjoint_backward", "JP_EULER", "JP_RK2", "JP_RK4", "HostParticles", "init_particles", "advection",
            "move_particles", "particle2grid", "particle2grid_multi", "grid2particle",
            "inject_particles", "lerp", "bisect", "first_stage", "second_stage", "philox",
            "uniform", "cell_index", "build", "lib"]

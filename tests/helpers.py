@@ -73,3 +73,33 @@ def clear(hp):
 
 def elem(hp, cell, slot):
     return int(np.ravel_multi_index(cell, hp.ni, order="F")) + hp.C * slot
+
+
+# ---- adjoint cases -------------------------------------------------------
+
+def flat_active(hp):
+    """Active particles as a flat list (slot-major storage order) and the mask."""
+    m = hp.index.astype(bool)
+    return [c[m].copy() for c in hp.coords], m
+
+
+def blob(x0):
+    return np.exp(-((x0[0] - 0.5) ** 2 + (x0[1] - 0.7) ** 2) / 0.02)
+
+
+def adjoint_case(D, outflow=False, npq=2):
+    """Small adjoint workload: host particles, velocity, time step.  With
+    `outflow` a uniform velocity carries part of the particles out."""
+    if D == 2:
+        xv, xi = grid2d(12, 10)
+        V = stream_V(xi)
+        hp = host_particles(xv, xi, 32, npq, seed=3)
+    else:
+        xv, xi = grid3d(5)
+        V = cellular_V(xi)
+        hp = host_particles(xv, xi, 40, npq, seed=3)
+    if outflow:
+        u = (3.0, 1.0, -0.5)
+        V = tuple(sample(lambda *X, c=c: np.full_like(X[0], u[c]), xi[c]) for c in range(D))
+    dt = cfl_dt(xv, V) * (0.2 if outflow else 0.7)
+    return hp, V, dt
