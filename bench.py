@@ -49,23 +49,42 @@ def algorithmic_bytes(S, P, D, M, F, Nn, Vb, injected=0, w=8):
             "step": adv + move + inj + p2g}
 
 
-def cpu_baseline(cfg, n_steps, seed):
+# The two single-grid workloads: 2D temperature advection (RK2, with
+# injection, Gaussian T from grid2particle!) and 3D RK4 advection of one
+# uniform-random particle field (no injection).
+WORKLOADS = {
+    2: {"nxcell": 24, "max_xcell": 48, "min_xcell": 12, "method": "rk2", "inject": True},
+    3: {"nxcell": 27, "max_xcell": 54, "min_xcell": 14, "method": "rk4", "inject": False},
+}
+
+
+def integrator(jp, spec):
+    return jp.RungeKutta2() if spec["method"] == "rk2" else jp.RungeKutta4()
+
+
+def cpu_baseline(cfg, spec, n_steps, seed):
     """The oracle (sequential C, 1 core) on a bounded sample of the same
     workload: n_steps full steps at the same size."""
     from oracle import oracle as O
     from justpic_hip.configs import gaussian_T
     xv, xi, V, dt = cfg["xv"], cfg["xi_vel"], cfg["V"], cfg["dt"]
-    hp = O.HostParticles(xv, 48, xi)
-    O.init_particles(hp, 6, seed)
-    T = gaussian_T(xv)
-    pT = O.grid2particle(hp, T, np.full(hp.S, np.nan))
+    D = len(xv)
+    nq = 2 ** D
+    hp = O.HostParticles(xv, spec["max_xcell"], xi)
+    O.init_particles(hp, spec["nxcell"] // nq, seed)
+    if D == 2:
+        pT = O.grid2particle(hp, gaussian_T(xv), np.full(hp.S, np.nan))
+    else:
+        pT = np.random.default_rng(seed).random(hp.S)
+    method = O.JP_RK2 if spec["method"] == "rk2" else O.JP_RK4
     psteps = 0
     t0 = time.perf_counter()
     for it in range(n_steps):
         psteps += int(hp.index.sum())
-        O.advection(hp, V, O.JP_RK2, 0.5, dt)
+        O.advection(hp, V, method, 0.5, dt)
         O.move_particles(hp, [pT])
-        O.inject_particles(hp, [pT], 12, seed, it)
+        if spec["inject"]:
+            O.inject_particles(hp, [pT], spec["min_xcell"], seed, it)
         O.particle2grid(hp, pT)
     el = time.perf_counter() - t0
     return psteps / el, el
@@ -86,29 +105,35 @@ def read_traffic(path):
     return None
 
 
-def run_form(jp, ctx, cfg, form, args):
+def run_form(jp, ctx, cfg, form, args, spec):
     """Warm up, then time args.steps full steps of one position form.  Every
     call is bracketed by hipEvents on the context stream."""
     from justpic_hip.configs import gaussian_T
     xv, xi, V, dt = cfg["xv"], cfg["xi_vel"], cfg["V"], cfg["dt"]
-    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed,
-                          form="split" if form == "split" else "f64")
+    D = len(xv)
+    p = jp.init_particles(ctx, spec["nxcell"], spec["max_xcell"], spec["min_xcell"], *xi,
+                          seed=args.seed, form="split" if form == "split" else "f64")
     Vd = tuple(jp.DeviceArray.from_host(ctx, v) for v in V)
-    T = jp.DeviceArray.from_host(ctx, gaussian_T(xv))
     pT = jp.CellArray(ctx, p.ni, p.max_xcell, fill=float("nan"))
-    jp.grid2particle_(pT, T, p)
-    rk2 = jp.RungeKutta2()
+    if D == 2:
+        T = jp.DeviceArray.from_host(ctx, gaussian_T(xv))
+        jp.grid2particle_(pT, T, p)
+    else:
+        T = jp.nodal_field(p)
+        pT.copy_from(np.random.default_rng(args.seed).random(p.C * p.max_xcell))
+    rk = integrator(jp, spec)
     ctx.sync()
-    calls = (lambda: jp.advection_(p, rk2, Vd, dt),
+    calls = (lambda: jp.advection_(p, rk, Vd, dt),
              lambda: jp.move_particles_(p, (pT,)),
-             lambda: jp.inject_particles_(p, (pT,)),
+             (lambda: jp.inject_particles_(p, (pT,))) if spec["inject"] else None,
              lambda: jp.particle2grid_(T, pT, p))
 
     def step(evs=None):
         for i, f in enumerate(calls):
             if evs is not None:
                 ctx.record(evs[2 * i])
-            f()
+            if f is not None:
+                f()
             if evs is not None:
                 ctx.record(evs[2 * i + 1])
 
@@ -139,29 +164,34 @@ def run_form(jp, ctx, cfg, form, args):
         f" P {P0}->{P1}, " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in call_ms.items()))
 
     # one calibration step with outcome counters (not timed) for the byte model
-    jp.advection_(p, rk2, Vd, dt)
+    jp.advection_(p, rk, Vd, dt)
     st_m = jp.move_particles_(p, (pT,), stats=True)
-    st_i = jp.inject_particles_(p, (pT,), stats=True)
+    st_i = jp.inject_particles_(p, (pT,), stats=True) if spec["inject"] else {"injected": 0}
     ctx.sync()
     S = p.C * p.max_xcell
     Nn = int(np.prod([n + 1 for n in p.ni]))
     Vb = sum(v.nbytes for v in Vd)
-    B = algorithmic_bytes(S, P, 2, st_m["moved"] + st_m["deleted"], 1, Nn, Vb,
+    B = algorithmic_bytes(S, P, D, st_m["moved"] + st_m["deleted"], 1, Nn, Vb,
                           st_i["injected"], w=4 if form == "split" else 8)
+    if not spec["inject"]:
+        B["step"] -= B["inject"]
+        B["inject"] = 0
     calls_out = {k: {"us": call_ms[k] * 1e3, "bytes": B[k],
-                     "GBs": B[k] / (call_ms[k] * 1e-3) / 1e9} for k in CALLS}
+                     "GBs": B[k] / (call_ms[k] * 1e-3) / 1e9} for k in CALLS
+                 if B[k] > 0}
     return {"value": value, "ms_per_step": ms_per_step, "P": P, "bytes": B,
             "calls": calls_out, "moved": st_m["moved"], "injected": st_i["injected"],
             "hbm_frac_step": B["step"] / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def split_error(jp, ctx, cfg, args):
+def split_error(jp, ctx, cfg, args, spec):
     """Advection + move for args.check_steps steps from one initial state in
     both forms, particles tracked by an id field: max position error of the
     FP32-split form against the FP64 form (bit-identical to the FP64 oracle in
     the parity tests), and the number of cell-assignment disagreements."""
     xi, V, dt = cfg["xi_vel"], cfg["V"], cfg["dt"]
-    p = jp.init_particles(ctx, 24, 48, 12, *xi, seed=args.seed)
+    p = jp.init_particles(ctx, spec["nxcell"], spec["max_xcell"], spec["min_xcell"], *xi,
+                          seed=args.seed)
     s = jp.to_split(p)
     Vd = tuple(jp.DeviceArray.from_host(ctx, v) for v in V)
     idx0 = p.index.data.flat()
@@ -170,11 +200,11 @@ def split_error(jp, ctx, cfg, args):
     id_p.copy_from(ids0)
     id_s = jp.CellArray(ctx, p.ni, p.max_xcell)
     id_s.copy_from(ids0)
-    rk2 = jp.RungeKutta2()
+    rk = integrator(jp, spec)
     for _ in range(args.check_steps):
-        jp.advection_(p, rk2, Vd, dt)
+        jp.advection_(p, rk, Vd, dt)
         jp.move_particles_(p, (id_p,))
-        jp.advection_(s, rk2, Vd, dt)
+        jp.advection_(s, rk, Vd, dt)
         jp.move_particles_(s, (id_s,))
     ctx.sync()
     C = p.C
@@ -269,20 +299,141 @@ def run_ensemble(jp, args):
     ens.close()
 
 
+def run_adjoint(jp, args):
+    """Adjoint inverse problem: 256^2 cells, nxcell 16 (max 32, min 8), true
+    velocity = rotating cell (amplitude 250) + 10% smooth divergence-free
+    perturbation, guess = the rotating cell, Gaussian tracer (sigma 0.1 at
+    (0.5, 0.7)), 100 RK2 steps, particle2grid! to the 257^2 nodes, L2 loss
+    against the true field's result.  One step is one gradient: the forward
+    pass (checkpoints every 10 steps), the loss cotangent, the pullback."""
+    from justpic_hip.configs import inverse2d
+    n, N, every = args.n or 256, 100, 10
+    ctx = jp.Context(0, synchronous=False)
+    cfg = inverse2d(n)
+    p = jp.init_particles(ctx, 16, 32, 8, *cfg["xi_vel"], seed=args.seed)
+    x0d, _ = jp.flatten(p)
+    x0 = [x.to_host() for x in x0d]
+    Tp = jp.DeviceArray.from_host(ctx, cfg["T"](x0[0], x0[1]))
+    P = Tp.size
+    Vg = tuple(jp.DeviceArray.from_host(ctx, v) for v in cfg["V"])
+    Vt = tuple(jp.DeviceArray.from_host(ctx, v) for v in cfg["V_true"])
+    rk2 = jp.RungeKutta2()
+    Ft = jp.AdvectionP2G(p, rk2, cfg["dt"], N, every).forward(Vt, x0d, Tp).flat()
+    op = jp.AdvectionP2G(p, rk2, cfg["dt"], N, every)
+    Fb = jp.DeviceArray(ctx, tuple(k + 1 for k in p.ni))
+    out = None
+    fw_ms, bw_ms, losses = [], [], []
+    e = [ctx.event() for _ in range(4)]
+    log("adjoint: particles", P, "steps", N)
+    for it in range(args.warmup + args.steps):
+        ctx.record(e[0])
+        F = op.forward(Vg, x0d, Tp)
+        ctx.record(e[1])
+        L, fb = jp.l2_loss(F, Ft)  # synchronises
+        Fb.copy_from(fb)
+        ctx.record(e[2])
+        out = op.pullback(Fb, out)
+        ctx.record(e[3])
+        ctx.sync()
+        if it >= args.warmup:
+            fw_ms.append(ctx.elapsed_ms(e[0], e[1]))
+            bw_ms.append(ctx.elapsed_ms(e[2], e[3]))
+            losses.append(L)
+    fw, bw = float(np.mean(fw_ms)), float(np.mean(bw_ms))
+    # inverse problem: a few steepest-descent steps with backtracking
+    V = [v.to_host() for v in Vg]
+    L0 = losses[0]
+    L, hist = L0, [L0]
+    for _ in range(args.descent):
+        g = [v.to_host() for v in out[0]]
+        eta = 0.05 * max(np.abs(v).max() for v in V) / max(np.abs(x).max() for x in g)
+        for _ in range(8):
+            Vn = [v - eta * x for v, x in zip(V, g)]
+            for d, v in zip(Vg, Vn):
+                d.copy_from(v)
+            Ln, fb = jp.l2_loss(op.forward(Vg, x0d, Tp), Ft)
+            if Ln < L:
+                break
+            eta *= 0.5
+        V, L = Vn, Ln
+        hist.append(L)
+        Fb.copy_from(fb)
+        out = op.pullback(Fb, out)
+    ctx.sync()
+    err_true = max(float(np.abs(v - t).max()) for v, t in zip(V, cfg["V_true"]))
+    log("loss history", hist)
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        hp = O.HostParticles(cfg["xv"], 32, cfg["xi_vel"])
+        stride = 64
+        xs = [x[::stride].copy() for x in x0]
+        ts = Tp.to_host()[::stride].copy()
+        t0 = time.perf_counter()
+        r = O.adjoint_forward(hp, cfg["V"], O.JP_RK2, 0.5, cfg["dt"], N, xs, ts)
+        O.adjoint_backward(hp, cfg["V"], O.JP_RK2, 0.5, cfg["dt"], N, xs, ts, r["F"], r["W"],
+                           np.nan_to_num(r["F"]))
+        el = time.perf_counter() - t0
+        cpu = {"value": xs[0].size * N / el, "unit": "gradient particle-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"every {stride}th particle ({xs[0].size}), {N} RK2 steps, forward + "
+                         f"backward by the sequential C oracle, {el:.1f} s"}
+    # forward-pass algorithmic bytes: x0, checkpoints, xN, alive, cell ids, bucket
+    # lists, V once, nodal F and W
+    nblk = -(-N // every)
+    Vb = sum(v.nbytes for v in Vg)
+    nodes = Fb.size
+    fw_bytes = P * (2 * 8 + nblk * 2 * 8 + 2 * 8 + 1 + 4 + 4) + Vb + 2 * 8 * nodes
+    res = {
+        "metric": "gradient particle-steps/s (adjoint of RK2 advection + particle2grid, FP64)",
+        "value": P * N / ((fw + bw) * 1e-3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fw + bw,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"adjoint inverse problem {n}^2 cells, nxcell 16, max_xcell 32, "
+                               f"{N} RK2 steps, checkpoints every {every}, L2 loss on the "
+                               f"{n + 1}^2 nodes",
+                   "particles": P, "forward_ms": fw, "backward_ms": bw,
+                   "loss_history": hist, "max_abs_V_error_vs_true": err_true,
+                   "initial_max_abs_V_error": max(float(np.abs(a - b).max()) for a, b in
+                                                  zip(cfg["V"], cfg["V_true"]))},
+        "roofline": {"bound": "hbm", "kernel": "adjoint forward pass (all launches)",
+                     "achieved": fw_bytes / (fw * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": fw_bytes / (fw * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": fw_bytes, "avg_launch_us": fw * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=512, help="cells per dimension")
+    ap.add_argument("--n", type=int, default=None,
+                    help="cells per dimension (default 512 step, 128 step3d, 256 adjoint)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--form", choices=("split", "fp64"), default="split",
                     help="position form whose throughput is the reported value")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the other form and the error check")
     ap.add_argument("--check-steps", type=int, default=50)
-    ap.add_argument("--case", choices=("step", "ensemble"), default="step",
-                    help="step: the 512^2 workload (default); ensemble: batched 2D grids")
+    ap.add_argument("--case", choices=("step", "step3d", "ensemble", "adjoint"),
+                    default="step",
+                    help="step: the 2D 512^2 workload (default); step3d: 3D RK4 128^3; "
+                         "ensemble: batched 2D grids; adjoint: gradient of the 256^2 "
+                         "inverse problem")
+    ap.add_argument("--descent", type=int, default=3,
+                    help="steepest-descent steps of the inverse problem (--case adjoint)")
     ap.add_argument("--members", type=int, default=64, help="ensemble size (--case ensemble)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
@@ -312,35 +463,61 @@ def main():
     if args.case == "ensemble":
         run_ensemble(jp, args)
         return
-
-    log("building workload", args.n)
-    cfg = stream2d(args.n, amp=250.0, readme=False)
+    if args.case == "adjoint":
+        run_adjoint(jp, args)
+        return
+    D = 3 if args.case == "step3d" else 2
+    spec = WORKLOADS[D]
+    args.n = args.n or (128 if D == 3 else 512)
+    log("building workload", D, "D", args.n)
+    if D == 2:
+        cfg = stream2d(args.n, amp=250.0, readme=False)
+    else:
+        from justpic_hip.configs import cellular3d
+        cfg = cellular3d(args.n, amp=100.0)
     ctx = jp.Context(0, synchronous=False)
-    main_r = run_form(jp, ctx, cfg, args.form, args)
+    main_r = run_form(jp, ctx, cfg, args.form, args, spec)
     other = None
     err = None
     if not args.no_compare:
         other_form = "fp64" if args.form == "split" else "split"
-        other = run_form(jp, ctx, cfg, other_form, args)
+        other = run_form(jp, ctx, cfg, other_form, args, spec)
         log("split vs fp64 error check,", args.check_steps, "steps")
-        err = split_error(jp, ctx, cfg, args)
+        err = split_error(jp, ctx, cfg, args, spec)
         log("error", err)
 
     # roofline: the advection kernel (one launch per call)
     adv = main_r["calls"]["advection"]
+    tag = f"{args.n}" if D == 2 else f"{args.n}_3d"
     traffic = read_traffic(os.path.join(ROOT, "profiles",
-                                        f"traffic_advection_{args.form}_{args.n}.txt"))
+                                        f"traffic_advection_{args.form}_{tag}.txt"))
+    meth = spec["method"].upper()
+    seq = f"advect {meth} + move" + (" + inject" if spec["inject"] else "") + " + p2g"
+    size = f"{args.n}^{D}"
 
     cpu = None
-    if args.cpu_steps > 0:
-        log("cpu baseline: oracle,", args.cpu_steps, "steps at the same size")
-        v_cpu, el = cpu_baseline(cfg, args.cpu_steps, args.seed)
+    # 3D: one step of the same workload at half the resolution (1/8 of the
+    # particles) keeps the sequential oracle near 10 s
+    cpu_steps = args.cpu_steps if D == 2 else min(args.cpu_steps, 1)
+    if cpu_steps > 0:
+        ccfg, csize = cfg, size
+        if D == 3:
+            from justpic_hip.configs import cellular3d
+            ccfg, csize = cellular3d(args.n // 2, amp=100.0), f"{args.n // 2}^3"
+        log("cpu baseline: oracle,", cpu_steps, "steps at", csize)
+        v_cpu, el = cpu_baseline(ccfg, spec, cpu_steps, args.seed)
         cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} full FP64 steps (advect RK2 + move + inject + p2g) "
-                         f"of the same {args.n}^2 workload by the sequential C oracle, "
-                         f"{el:.1f} s"}
+               "sample": f"{cpu_steps} full FP64 steps ({seq}) of the {csize} workload "
+                         f"by the sequential C oracle, {el:.1f} s"}
 
     names = {"split": "FP32-split positions", "fp64": "FP64 drop-in"}
+    if D == 2:
+        workload = (f"2D temperature advection {size} cells, nxcell 24, max_xcell 48, "
+                    f"min_xcell 12, RK2 + move + inject + p2g, {names[args.form]}")
+    else:
+        workload = (f"3D RK4 advection {size} cells, cellular flow x100, nxcell 27, "
+                    f"max_xcell 54, min_xcell 14, RK4 + move + p2g of one field, "
+                    f"{names[args.form]}")
     out = {
         "metric": METRIC,
         "value": main_r["value"],
@@ -354,15 +531,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if args.form == "split" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"2D temperature advection {args.n}^2 cells, nxcell 24, "
-                               f"max_xcell 48, min_xcell 12, RK2 + move + inject + p2g, "
-                               f"{names[args.form]}",
-                   "cells": args.n * args.n, "particles": int(main_r["P"]),
-                   "form": args.form},
+        "config": {"workload": workload, "cells": args.n ** D,
+                   "particles": int(main_r["P"]), "form": args.form},
         "hbm_frac_step": main_r["hbm_frac_step"],
         "step_bytes": main_r["bytes"]["step"],
         "calls": main_r["calls"],
-        "roofline": {"bound": "hbm", "kernel": f"advection RK2 ({args.form})",
+        "roofline": {"bound": "hbm", "kernel": f"advection {meth} ({args.form})",
                      "achieved": adv["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": adv["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": adv["bytes"], "avg_launch_us": adv["us"]},

@@ -71,7 +71,9 @@ class AdvectionP2G:
         self._fw = None
 
     def forward(self, V, x0, Tp) -> DeviceArray:
-        """Run the forward pass; returns the nodal field F (shape ni+1)."""
+        """Run the forward pass; returns the nodal field F (shape ni+1).  The
+        buffers of one object are reused: the next forward pass with the same
+        particle count overwrites F and the checkpoints."""
         ctx, D = self.ctx, self.D
         V = tuple(_as_device(ctx, v) for v in V)
         x0 = tuple(_as_device(ctx, x) for x in x0)
@@ -86,11 +88,15 @@ class AdvectionP2G:
         if any(x.size != n for x in x0):
             raise ValueError("x0 and Tp sizes differ")
         nblk = max(1, -(-self.nsteps // self.every))
-        ckpt = DeviceArray(ctx, (nblk * D * max(n, 1),))
-        alive = DeviceArray(ctx, (max(n, 1),), np.uint8)
-        xN = tuple(DeviceArray(ctx, (max(n, 1),)) for _ in range(D))
-        F = DeviceArray(ctx, self.nodes)
-        W = DeviceArray(ctx, self.nodes)
+        if self._fw is None or self._fw["n"] != n:
+            ckpt = DeviceArray(ctx, (nblk * D * max(n, 1),))
+            alive = DeviceArray(ctx, (max(n, 1),), np.uint8)
+            xN = tuple(DeviceArray(ctx, (max(n, 1),)) for _ in range(D))
+            F = DeviceArray(ctx, self.nodes)
+            W = DeviceArray(ctx, self.nodes)
+        else:  # the same buffers: a new forward pass overwrites the previous one
+            fw = self._fw
+            ckpt, alive, xN, F, W = fw["ckpt"], fw["alive"], fw["xN"], fw["F"], fw["W"]
         ctx.check(ctx.lib.jp_adjoint_forward(
             ctx.h, self.p.grid_ref(), self.p.velgrid_ref(), _abi.ptr_array([v.ptr for v in V]),
             self.method.code, self.method.alpha, self.dt, self.nsteps, self.every, n,
@@ -111,10 +117,11 @@ class AdvectionP2G:
     def W(self) -> DeviceArray:
         return self._fw["W"]
 
-    def pullback(self, Fbar):
+    def pullback(self, Fbar, out=None):
         """Cotangents ``(Vbar, x0bar, Tpbar)`` for the cotangent ``Fbar`` of F
         (finite everywhere: give 0 at nodes the loss ignores, e.g. NaN nodes).
-        Deleted particles get zero cotangents."""
+        Deleted particles get zero cotangents.  ``out`` (a previous result)
+        is overwritten instead of allocating new arrays."""
         if self._fw is None:
             raise RuntimeError("pullback before forward")
         ctx, D, fw = self.ctx, self.D, self._fw
@@ -122,9 +129,15 @@ class AdvectionP2G:
         if Fbar.size != fw["F"].size:
             raise ValueError("Fbar must have the nodal shape ni+1")
         n = fw["n"]
-        Vbar = tuple(DeviceArray(ctx, v.shape) for v in fw["V"])
-        x0bar = tuple(DeviceArray(ctx, (max(n, 1),)) for _ in range(D))
-        Tpbar = DeviceArray(ctx, (max(n, 1),))
+        if out is None:
+            Vbar = tuple(DeviceArray(ctx, v.shape) for v in fw["V"])
+            x0bar = tuple(DeviceArray(ctx, (max(n, 1),)) for _ in range(D))
+            Tpbar = DeviceArray(ctx, (max(n, 1),))
+        else:
+            Vbar, x0bar, Tpbar = out
+            if (any(a.size != v.size for a, v in zip(Vbar, fw["V"]))
+                    or any(x.size < n for x in x0bar) or Tpbar.size < n):
+                raise ValueError("out arrays do not match this forward pass")
         ctx.check(ctx.lib.jp_adjoint_backward(
             ctx.h, self.p.grid_ref(), self.p.velgrid_ref(),
             _abi.ptr_array([v.ptr for v in fw["V"]]), self.method.code, self.method.alpha,
