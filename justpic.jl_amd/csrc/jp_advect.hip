@@ -1,10 +1,13 @@
 // jp_advect.hip -- advection! (src/Particles/Advection/advection.jl:35-91) on
 // gfx950, FP64 drop-in, 2D and 3D, Euler / RK2 / RK4.
 //
-// One workgroup per tile of cells with the tile's velocity and axis windows
-// in LDS (see the tiled kernel below); a bracket lookup costs one guess
-// (uniform spacing estimate) and one strict test against the stored axis, with
-// the reference search for everything else (ties, refined axes).
+// One thread per slot; the velocity components are gathered from global
+// memory, where the few MB of a velocity grid stay L2/MALL-resident.  A bracket
+// lookup costs one guess (uniform spacing estimate) and one strict test against
+// the stored axis, with the reference search for everything else (ties,
+// refined axes).  An LDS-tiled variant (velocity and axis windows per tile of
+// cells, a thread per cell walking its slots) was as fast in 2D and half as
+// fast in 3D, where its register footprint left one wave per SIMD.
 #include "jp_ctx.hpp"
 
 using namespace jp;
@@ -30,114 +33,51 @@ struct RKParams {
 // n-linear lerp of the corner values (src/Interpolations/ndlerp.jl:11-15) in
 // extract_field_corners order (src/Advection/advection.jl:47-71).
 
-// ---------------------------------------------------------------- tiled kernel
-// One workgroup per tile of cells (2D 64 x 4, 3D 16 x 4 x 4), one thread per
-// cell, slots walked in chunks of 8 planes in lockstep across the wavefront.
-// Per tile, LDS holds for every velocity component c the corner values of the
-// node window [t0 - 1, t0 + T + 3) per dimension, and for every axis (c, d) the
-// coordinates and inverse spacings of the same node window.  A bracket is the
-// uniform-spacing guess confirmed by the strict test x[k] < p < x[k+1] on the
-// stored coordinates (a strictly interior point has one bracket, the one the
-// reference's bisection returns whatever its seed); ties and misses take the
-// seeded bisection on the global axis.  Values read from LDS are the stored
-// values, so results are identical to the global-memory path.
+// Row mapping: a wavefront holds 64 consecutive cells of one grid row at one
+// slot plane (grid.y), so coordinate and mask accesses coalesce.  Brackets:
+// the uniform-spacing guess confirmed with the strict test x[k] < p < x[k+1]
+// (a strictly interior point has one bracket, the one the reference's
+// bisection returns whatever its seed), else the reference search seeded at
+// the particle's cell.
 template <int D>
-__host__ __device__ constexpr int tile_n(int d) {
-    return D == 2 ? (d == 0 ? 64 : 4) : (d == 0 ? 16 : 4);
-}
-template <int D>
-__host__ __device__ constexpr int win_n(int d) {
-    return tile_n<D>(d) + 4;
-}
-template <int D>
-__host__ __device__ constexpr int win_size() {
-    return D == 2 ? win_n<2>(0) * win_n<2>(1) : win_n<3>(0) * win_n<3>(1) * win_n<3>(2);
-}
-template <int D>
-__host__ __device__ constexpr int win_max() {
-    return win_n<D>(0);
-}
-
-template <int D>
-struct TileLDS {
-    double V[D][win_size<D>()];
-    double x[D][D][win_max<D>()];    // axis (c, d): x[k], k = t0[d] - 1 + local
-    double idx[D][D][win_max<D>()];  // 1 / dx[k]
-    double lim[D][D][2];             // first and last coordinate of axis (c, d)
-    double g0[D][D][2];              // x[0] and inv_dx[0]: the uniform guess
+struct AxisGuess {
+    double lo[D][D], hi[D][D], i0[D][D];  // x[0], x[n-1], inv_dx[0] of axis (c, d)
 };
 
-// ties and guesses outside the window: the seeded search on the global axis
-__device__ inline void bracket_global(double p, const Axis& A, int64_t seed, int64_t& k,
-                                           double& xk, double& ik) {
-    k = find_cell(p, A, seed);
-    xk = A.x[k];
-    ik = A.idx[k];
-}
-
-// bracket of p on axis (c, d): global node k, its coordinate and inverse spacing
-template <int D>
-__device__ inline void bracket_tile(const TileLDS<D>& L, const Axis& A, int c, int d, int base,
-                                    double p, int64_t seed, int64_t& k, double& xk, double& ik,
-                                    bool& inw) {
+__device__ inline int64_t bracket_slot(double p, const Axis& A, double x0, double i0,
+                                       int64_t seed) {
     const int64_t last = A.n - 2;
-    int64_t gk = (int64_t)((p - L.g0[c][d][0]) * L.g0[c][d][1]);
+    int64_t gk = (int64_t)((p - x0) * i0);
     gk = gk < 0 ? 0 : (gk > last ? last : gk);
-    const int64_t kl = gk - base;
-    if (kl >= 0 && kl + 1 < win_n<D>(d)) {
-        const double a = L.x[c][d][kl], b = L.x[c][d][kl + 1];
-        if (a < p && p < b) {
-            k = gk;
-            xk = a;
-            ik = L.idx[c][d][kl];
-            inw = true;
-            return;
-        }
-    }
-    bracket_global(p, A, seed, k, xk, ik);
-    const int64_t kk = k - base;
-    inw = kk >= 0 && kk + 1 < win_n<D>(d);
+    if (A.x[gk] < p && p < A.x[gk + 1]) return gk;
+    return find_cell(p, A, seed);
 }
 
 template <int D>
-__device__ inline double interp_tile(const TileLDS<D>& L, const Vel<D>& vg, const VPtrs<D>& V,
-                                     int c, const int (&base)[D], const double (&p)[D],
+__device__ inline double interp_slot(const AxisGuess<D>& Gs, const Vel<D>& vg,
+                                     const VPtrs<D>& V, int c, const double (&p)[D],
                                      const int64_t (&I)[D]) {
     bool in = true;
 #pragma unroll
-    for (int d = 0; d < D; ++d) in = in && (L.lim[c][d][0] <= p[d] && p[d] <= L.lim[c][d][1]);
+    for (int d = 0; d < D; ++d) in = in && (Gs.lo[c][d] <= p[d] && p[d] <= Gs.hi[c][d]);
     if (!in) return __builtin_inf();
     int64_t k[D];
     double t[D];
-    bool inw = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        double xk, ik;
-        bool w;
-        bracket_tile<D>(L, vg.ax[c][d], c, d, base[d], p[d], I[d], k[d], xk, ik, w);
-        t[d] = (p[d] - xk) * ik;
-        inw = inw && w;
+        const Axis& A = vg.ax[c][d];
+        k[d] = bracket_slot(p[d], A, Gs.lo[c][d], Gs.i0[c][d], I[d]);
+        t[d] = (p[d] - A.x[k[d]]) * A.idx[k[d]];
     }
+    const double* __restrict__ F = V.v[c];
+    const int64_t n0 = vg.ax[c][0].n, n1 = vg.ax[c][1].n;
+    int64_t b = k[0] + n0 * k[1];
+    if constexpr (D == 3) b += n0 * n1 * k[2];
     double q[1 << D];
-    if (inw) {
-        constexpr int W0 = win_n<D>(0), W1 = win_n<D>(1);
-        int b = (int)(k[0] - base[0]) + W0 * (int)(k[1] - base[1]);
-        if constexpr (D == 3) b += W0 * W1 * (int)(k[2] - base[2]);
 #pragma unroll
-        for (int r = 0; r < (1 << D); ++r) {
-            const int o = (r & 1) + ((r >> 1) & 1) * W0 + (D == 3 ? ((r >> 2) & 1) * W0 * W1 : 0);
-            q[r] = L.V[c][b + o];
-        }
-    } else {
-        const double* __restrict__ F = V.v[c];
-        const int64_t n0 = vg.ax[c][0].n, n1 = vg.ax[c][1].n;
-        int64_t b = k[0] + n0 * k[1];
-        if constexpr (D == 3) b += n0 * n1 * k[2];
-#pragma unroll
-        for (int r = 0; r < (1 << D); ++r) {
-            const int64_t o = (r & 1) + ((r >> 1) & 1) * n0 + (D == 3 ? ((r >> 2) & 1) * n0 * n1 : 0);
-            q[r] = F[b + o];
-        }
+    for (int r = 0; r < (1 << D); ++r) {
+        const int64_t o = (r & 1) + ((r >> 1) & 1) * n0 + (D == 3 ? ((r >> 2) & 1) * n0 * n1 : 0);
+        q[r] = F[b + o];
     }
     const double lo = lerp1(t[1], lerp1(t[0], q[0], q[1]), lerp1(t[0], q[2], q[3]));
     if constexpr (D == 2) {
@@ -149,22 +89,20 @@ __device__ inline double interp_tile(const TileLDS<D>& L, const Vel<D>& vg, cons
 }
 
 template <int D>
-__device__ inline void interp_v_tile(const TileLDS<D>& L, const Vel<D>& vg, const VPtrs<D>& V,
-                                     const int (&base)[D], const double (&p)[D],
-                                     const int64_t (&I)[D], double (&v)[D]) {
+__device__ inline void interp_v_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, const VPtrs<D>& V,
+                                     const double (&p)[D], const int64_t (&I)[D],
+                                     double (&v)[D]) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) v[c] = interp_tile<D>(L, vg, V, c, base, p, I);
+    for (int c = 0; c < D; ++c) v[c] = interp_slot<D>(Gs, vg, V, c, p, I);
 }
 
-// advect_particle with the tile's LDS: Euler (Euler.jl:1-20), RK2 (RK2.jl:1-26,
-// stages src/Advection/RK2.jl:1-44), RK4 (RK4.jl:1-19, no muladd)
+// advect_particle (Euler.jl, RK2.jl, RK4.jl) with slot-kernel interpolation
 template <int D, int M>
-__device__ inline void advect_tile(const TileLDS<D>& L, const Vel<D>& vg, const VPtrs<D>& V,
-                                   const RKParams& P, const int (&base)[D],
-                                   const double (&p0)[D], const int64_t (&I)[D],
-                                   double (&out)[D]) {
+__device__ inline void advect_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, const VPtrs<D>& V,
+                                   const RKParams& P, const double (&p0)[D],
+                                   const int64_t (&I)[D], double (&out)[D]) {
     double v0[D];
-    interp_v_tile<D>(L, vg, V, base, p0, I, v0);
+    interp_v_slot<D>(Gs, vg, V, p0, I, v0);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v0[d], p0[d]);
@@ -172,7 +110,7 @@ __device__ inline void advect_tile(const TileLDS<D>& L, const Vel<D>& vg, const 
         double p1[D], v1[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) p1[d] = fma(P.adt, v0[d], p0[d]);
-        interp_v_tile<D>(L, vg, V, base, p1, I, v1);  // same seed I (RK2.jl:14,20)
+        interp_v_slot<D>(Gs, vg, V, p1, I, v1);  // same seed I (RK2.jl:14,20)
         if (P.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v1[d], p0[d]);
@@ -181,149 +119,52 @@ __device__ inline void advect_tile(const TileLDS<D>& L, const Vel<D>& vg, const 
             for (int d = 0; d < D; ++d) out[d] = fma(P.dt, fma(P.w1, v1[d], P.w0 * v0[d]), p0[d]);
         }
     } else {
-        double q[D], k2[D], k3[D], k4[D];
+        double q[D], k[D], sum[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * v0[d]) / 2.0;
-        interp_v_tile<D>(L, vg, V, base, q, I, k2);
-#pragma unroll
-        for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * k2[d]) / 2.0;
-        interp_v_tile<D>(L, vg, V, base, q, I, k3);
-#pragma unroll
-        for (int d = 0; d < D; ++d) q[d] = p0[d] + P.dt * k3[d];
-        interp_v_tile<D>(L, vg, V, base, q, I, k4);
+        interp_v_slot<D>(Gs, vg, V, q, I, k);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const double sum = ((v0[d] + 2.0 * k2[d]) + 2.0 * k3[d]) + k4[d];
-            out[d] = p0[d] + (P.dt * sum) / 6.0;
+            sum[d] = v0[d] + 2.0 * k[d];
+            q[d] = p0[d] + (P.dt * k[d]) / 2.0;
         }
+        interp_v_slot<D>(Gs, vg, V, q, I, k);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            sum[d] = sum[d] + 2.0 * k[d];
+            q[d] = p0[d] + P.dt * k[d];
+        }
+        interp_v_slot<D>(Gs, vg, V, q, I, k);
+#pragma unroll
+        for (int d = 0; d < D; ++d) out[d] = p0[d] + (P.dt * (sum[d] + k[d])) / 6.0;
     }
 }
 
-// advection_kernel! (advection.jl:66-91): doskip, advect, write back
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) k_advect(Grid<D> g, Vel<D> vg, VPtrs<D> V, Ptr3 coords,
-                                                   const uint8_t* __restrict__ index, RKParams P) {
-    __shared__ TileLDS<D> L;
-    int t0[D], base[D];
-    {
-        int64_t b = blockIdx.x;
+__global__ void __launch_bounds__(kBlock) k_advect_slot(Grid<D> g, Vel<D> vg, VPtrs<D> V,
+                                                        Ptr3 coords,
+                                                        const uint8_t* __restrict__ index,
+                                                        RKParams P) {
+    int64_t c, I[D];
+    if (!row_cell<D>(g, c, I)) return;
+    const int64_t e = c + g.C * (int64_t)blockIdx.y;
+    if (!index[e]) return;  // doskip (src/Utils.jl:32-33)
+    AxisGuess<D> Gs;
+#pragma unroll
+    for (int cc = 0; cc < D; ++cc)
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int64_t nt = (g.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
-            t0[d] = (int)(b % nt) * tile_n<D>(d);
-            b /= nt;
-            base[d] = t0[d] - 1;
+            const Axis& A = vg.ax[cc][d];
+            Gs.lo[cc][d] = A.x[0];
+            Gs.hi[cc][d] = A.x[A.n - 1];
+            Gs.i0[cc][d] = A.idx[0];
         }
-    }
-    const int tid = threadIdx.x;
-    // window fills: every load issued before the LDS stores
-    constexpr int NI = (win_size<D>() + kBlock - 1) / kBlock;
-    double wv[D][NI];
+    double p0[D], out[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-        const double* __restrict__ F = V.v[c];
+    for (int d = 0; d < D; ++d) p0[d] = coords.p[d][e];
+    advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
 #pragma unroll
-        for (int it = 0; it < NI; ++it) {
-            const int i = tid + it * kBlock;
-            int r = i;
-            int64_t node = 0, stride = 1;
-            bool ok = i < win_size<D>();
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int64_t k = base[d] + r % win_n<D>(d);
-                r /= win_n<D>(d);
-                ok = ok && k >= 0 && k < vg.ax[c][d].n;
-                node += k * stride;
-                stride *= vg.ax[c][d].n;
-            }
-            wv[c][it] = ok ? F[ok ? node : 0] : 0.0;
-        }
-    }
-    constexpr int NA = D * D * win_max<D>();
-    constexpr int NJ = (NA + kBlock - 1) / kBlock;
-    double ax[NJ], ai[NJ];
-#pragma unroll
-    for (int it = 0; it < NJ; ++it) {
-        const int i = tid + it * kBlock;
-        const int c = i / (D * win_max<D>());
-        const int d = (i / win_max<D>()) % D;
-        const int j = i % win_max<D>();
-        const bool in = i < NA;
-        const Axis& A = vg.ax[in ? c : 0][in ? d : 0];
-        const int64_t k = base[in ? d : 0] + j;
-        const bool okx = in && j < win_n<D>(d) && k >= 0 && k < A.n;
-        const bool oki = in && j < win_n<D>(d) && k >= 0 && k < A.n - 1;
-        ax[it] = okx ? A.x[okx ? k : 0] : __builtin_nan("");
-        ai[it] = oki ? A.idx[oki ? k : 0] : __builtin_nan("");
-    }
-#pragma unroll
-    for (int c = 0; c < D; ++c)
-#pragma unroll
-        for (int it = 0; it < NI; ++it)
-            if (tid + it * kBlock < win_size<D>()) L.V[c][tid + it * kBlock] = wv[c][it];
-#pragma unroll
-    for (int it = 0; it < NJ; ++it) {
-        const int i = tid + it * kBlock;
-        if (i < NA) {
-            const int c = i / (D * win_max<D>());
-            const int d = (i / win_max<D>()) % D;
-            const int j = i % win_max<D>();
-            L.x[c][d][j] = ax[it];
-            L.idx[c][d][j] = ai[it];
-        }
-    }
-    if (tid < D * D) {
-        const int c = tid / D, d = tid % D;
-        const Axis& A = vg.ax[c][d];
-        L.lim[c][d][0] = A.x[0];
-        L.lim[c][d][1] = A.x[A.n - 1];
-        L.g0[c][d][0] = A.x[0];
-        L.g0[c][d][1] = A.idx[0];
-    }
-    __syncthreads();
-    int64_t I[D];
-    bool in = true;
-    {
-        int r = tid;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            I[d] = t0[d] + r % tile_n<D>(d);
-            r /= tile_n<D>(d);
-            in = in && I[d] < g.ni[d];
-        }
-    }
-    if (!in) return;
-    const int64_t c = lin<D>(g.ni, I);
-    const int64_t C = g.C;
-    const uint64_t act = active_slots(index, c, C, g.K);
-    constexpr int B = 8;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
-        const uint32_t ab = (uint32_t)(act >> k0) & 0xFFu;
-        if (!ab) continue;
-        double pb[B][D];
-#pragma unroll
-        for (int j = 0; j < B; ++j)
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                pb[j][d] = ((ab >> j) & 1) ? coords.p[d][c + C * (k0 + j)] : 0.0;
-        // one particle at a time (the chunk's loads are already in flight)
-        uint32_t todo = ab;
-        while (todo) {
-            const int j = __builtin_ctz(todo);
-            todo &= todo - 1;
-            double p0[D], out[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                double v = pb[0][d];
-#pragma unroll
-                for (int q = 1; q < B; ++q) v = j == q ? pb[q][d] : v;
-                p0[d] = v;
-            }
-            advect_tile<D, M>(L, vg, V, P, base, p0, I, out);
-#pragma unroll
-            for (int d = 0; d < D; ++d) coords.p[d][c + C * (k0 + j)] = out[d];
-        }
-    }
+    for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
 }
 
 template <int D>
@@ -340,9 +181,8 @@ Vel<D> to_vel(const jp_velgrid* vg) {
 template <int D, int M>
 int launch(jp_ctx* ctx, const Grid<D>& G, const Vel<D>& vg, const VPtrs<D>& V, Ptr3 cp,
            const uint8_t* index, const RKParams& P) {
-    int64_t tiles = 1;
-    for (int d = 0; d < D; ++d) tiles *= (G.ni[d] + tile_n<D>(d) - 1) / tile_n<D>(d);
-    k_advect<D, M><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P);
+    dim3 grid((unsigned)row_blocks<D>(G), (unsigned)G.K);
+    k_advect_slot<D, M><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P);
     JP_LAUNCHED(ctx);
     return JP_OK;
 }

@@ -356,63 +356,40 @@ __global__ void __launch_bounds__(kBlock) k_adv_split_ens(const EnsGrid* __restr
 
 // ------------------------------------------------------------ particle2grid
 // Inverse-distance-squared weights in float32 from the offsets (hardware
-// reciprocal, about 1 ulp), weight and weighted-field sums in FP64; cells
-// and slots in the reference order.
+// reciprocal, about 1 ulp), weight and weighted-field sums in FP64.  Two
+// passes, so that every particle is read once (a per-node gather would read
+// it from each of its cell's 2^D nodes):
+//  - k_p2g_split_cells: one thread per cell (row mapping: a wavefront's 64
+//    cells share each slot plane, so the loads coalesce) walks its active
+//    slots in chunks, the next chunk's loads in flight, and accumulates
+//    sum(w) and sum(w F) for each of the cell's 2^D corners; the partial sums
+//    go to part[(2 q + {0, 1}) C + c];
+//  - k_p2g_split_nodes: one thread per node adds the partials of its 2^D
+//    cells (in the reference's cell order) and divides.  A node without
+//    particles gets 0/0 = NaN.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_p2g_split(Grid<D> gg, SGrid<D> g, double* __restrict__ F,
-                                                      const double* __restrict__ Fp, FPtr3 off,
-                                                      const uint64_t* __restrict__ occw) {
-    int64_t nn[D];
-    int64_t NN = 1;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        nn[d] = g.ni[d] + 1;
-        NN *= nn[d];
-    }
-    const int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (node >= NN) return;
-    int N[D];
-    {
-        int64_t r = node;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            N[d] = (int)(r % nn[d]);
-            r /= nn[d];
-        }
-    }
+__global__ void __launch_bounds__(kBlock) k_p2g_split_cells(Grid<D> gg, SGrid<D> g,
+                                                            const double* __restrict__ Fp,
+                                                            FPtr3 off,
+                                                            const uint8_t* __restrict__ index,
+                                                            double* __restrict__ part) {
+    int64_t c, I[D];
+    if (!row_cell<D>(gg, c, I)) return;
+    const int64_t C = g.C;
+    const int K = g.K;
+    const uint64_t act = active_slots(index, c, C, K);
     float hf[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) hf[d] = (float)g.h[d][0];
-    const int64_t C = g.C;
     constexpr int NC = 1 << D;
-    int64_t cell[NC];
-    uint64_t bits[NC];
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-        int64_t I[D];
-        bool ok = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            I[d] = N[d] + (((q >> d) & 1) ? 0 : -1);
-            ok = ok && (I[d] >= 0 && I[d] < g.ni[d]);
-        }
-        cell[q] = ok ? lin<D>(gg.ni, I) : 0;
-        bits[q] = ok ? occw[cell[q]] : 0ull;
-    }
-    double W = 0.0, WF = 0.0;
-    // node minus cell corner in cells (the node is corner 1 - bit of cell q)
     constexpr int B = D == 2 ? 8 : 4;
-    const int nk = (g.K + B - 1) / B;
-    // the (cell, chunk) walk with the next chunk's loads in flight
-    auto load = [&](int q, int k0, float (&ub)[B][D], double (&fb)[B]) __attribute__((always_inline)) {
-        int64_t c = cell[0];
-        uint64_t bw = bits[0];
+    double W[NC], S[NC];
 #pragma unroll
-        for (int u = 1; u < NC; ++u) {
-            c = q == u ? cell[u] : c;
-            bw = q == u ? bits[u] : bw;
-        }
-        const uint32_t bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
+    for (int q = 0; q < NC; ++q) W[q] = S[q] = 0.0;
+    // inactive slots load offset 2 in every dimension (so s > 0) and add a
+    // zero weight: the accumulation is branch-free
+    auto load = [&](int k0, float (&ub)[B][D], double (&fb)[B]) __attribute__((always_inline)) {
+        const uint32_t bq = (uint32_t)(act >> k0) & ((1u << B) - 1);
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (bq >> j) & 1;
@@ -422,57 +399,101 @@ __global__ void __launch_bounds__(kBlock) k_p2g_split(Grid<D> gg, SGrid<D> g, do
             fb[j] = on ? Fp[e] : 0.0;
         }
     };
-    // a weight from an inactive slot (offset 2 in every dimension, so s > 0)
-    // is added as zero: the accumulation is branch-free
-    auto accumulate = [&](int q, int k0, const float (&ub)[B][D], const double (&fb)[B])
+    auto accumulate = [&](int k0, const float (&ub)[B][D], const double (&fb)[B])
                           __attribute__((always_inline)) {
-        uint64_t bw = bits[0];
-#pragma unroll
-        for (int u = 1; u < NC; ++u) bw = q == u ? bits[u] : bw;
-        const uint32_t bq = (uint32_t)(bw >> k0) & ((1u << B) - 1);
-        float a[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) a[d] = ((q >> d) & 1) ? 0.0f : 1.0f;
+        const uint32_t bq = (uint32_t)(act >> k0) & ((1u << B) - 1);
 #pragma unroll
         for (int j = 0; j < B; ++j) {
-            float s = 0.0f;
+            const bool on = (bq >> j) & 1;
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const float r = (a[d] - ub[j][d]) * hf[d];
-                s = fmaf(r, r, s);
+            for (int q = 0; q < NC; ++q) {
+                float sq = 0.0f;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const float r = ((float)((q >> d) & 1) - ub[j][d]) * hf[d];
+                    sq = fmaf(r, r, sq);
+                }
+                const double w = on ? (double)__builtin_amdgcn_rcpf(sq) : 0.0;
+                W[q] += w;
+                S[q] = fma(w, fb[j], S[q]);
             }
-            const double w = ((bq >> j) & 1) ? (double)__builtin_amdgcn_rcpf(s) : 0.0;
-            W += w;
-            WF = fma(w, fb[j], WF);
         }
     };
-    float u0[B][D], u1[B][D];
-    double f0[B], f1[B];
-    int q = 0, k0 = 0;
-    load(0, 0, u0, f0);
-    const int total = NC * nk;
+    // chunks that still hold an active slot
+    auto more = [&](int k) __attribute__((always_inline)) {
+        return k < K && (act >> k) != 0ull;
+    };
+    if (act != 0ull) {
+        float u0[B][D], u1[B][D];
+        double f0[B], f1[B];
+        int k0 = 0;
+        load(0, u0, f0);
 #pragma unroll 1
-    for (int t = 0; t < total; t += 2) {
-        // next chunk: (q, k0 + B), wrapping to the next cell
-        int q1 = q, k1 = k0 + B;
-        if (k1 >= nk * B) {
-            q1 = q + 1;
-            k1 = 0;
+        for (;;) {
+            const int k1 = k0 + B;
+            const bool m1 = more(k1);
+            if (m1) load(k1, u1, f1);
+            accumulate(k0, u0, f0);
+            if (!m1) break;
+            const int k2 = k1 + B;
+            const bool m2 = more(k2);
+            if (m2) load(k2, u0, f0);
+            accumulate(k1, u1, f1);
+            if (!m2) break;
+            k0 = k2;
         }
-        if (t + 1 < total) load(q1, k1, u1, f1);
-        accumulate(q, k0, u0, f0);
-        if (t + 1 >= total) break;
-        int q2 = q1, k2 = k1 + B;
-        if (k2 >= nk * B) {
-            q2 = q1 + 1;
-            k2 = 0;
-        }
-        if (t + 2 < total) load(q2, k2, u0, f0);
-        accumulate(q1, k1, u1, f1);
-        q = q2;
-        k0 = k2;
     }
-    F[node] = WF / W;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        part[(2 * q) * C + c] = W[q];
+        part[(2 * q + 1) * C + c] = S[q];
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_p2g_split_nodes(Grid<D> gg, SGrid<D> g,
+                                                            const double* __restrict__ part,
+                                                            double* __restrict__ F) {
+    int64_t nn[D];
+    int64_t NN = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        nn[d] = g.ni[d] + 1;
+        NN *= nn[d];
+    }
+    const int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (node >= NN) return;
+    int64_t N[D];
+    {
+        int64_t r = node;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            N[d] = r % nn[d];
+            r /= nn[d];
+        }
+    }
+    const int64_t C = g.C;
+    double W = 0.0, S = 0.0;
+    // cells (N - 1 .. N) per dimension, dimension 0 fastest; the node is the
+    // lower corner (bit 0) of cell N and the upper corner (bit 1) of cell N - 1
+#pragma unroll
+    for (int q = 0; q < (1 << D); ++q) {
+        int64_t I[D];
+        bool ok = true;
+        int cq = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const bool lower = (q >> d) & 1;
+            I[d] = lower ? N[d] : N[d] - 1;
+            ok = ok && I[d] >= 0 && I[d] < g.ni[d];
+            cq |= (lower ? 0 : 1) << d;
+        }
+        if (!ok) continue;
+        const int64_t cell = lin<D>(gg.ni, I);
+        W += part[(2 * cq) * C + cell];
+        S += part[(2 * cq + 1) * C + cell];
+    }
+    F[node] = S / W;
 }
 
 // ------------------------------------------------------------ grid2particle
@@ -967,10 +988,12 @@ int run_p2g(jp_ctx* ctx, const jp_cellgrid* gc, double* F, const double* Fp, flo
     int64_t NN = 1;
     for (int d = 0; d < D; ++d) NN *= G.ni[d] + 1;
     int rc;
-    if ((rc = ensure_scratch(ctx, (size_t)G.C * 8))) return rc;
-    uint64_t* occ = static_cast<uint64_t*>(ctx->scratch);
-    if ((rc = occ_words(ctx, index, G.C, G.K, occ))) return rc;
-    k_p2g_split<D><<<blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, F, Fp, fptr3<D>(off), occ);
+    if ((rc = ensure_scratch(ctx, (size_t)G.C * 8 * 2 * (1 << D)))) return rc;
+    double* part = static_cast<double*>(ctx->scratch);
+    k_p2g_split_cells<D><<<(unsigned)row_blocks<D>(G), kBlock, 0, ctx->stream>>>(
+        G, sg, Fp, fptr3<D>(off), index, part);
+    JP_LAUNCHED(ctx);
+    k_p2g_split_nodes<D><<<(unsigned)blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, part, F);
     JP_LAUNCHED(ctx);
     return finish(ctx);
 }
