@@ -84,9 +84,10 @@ __device__ inline int phase_of(const int64_t* I) {
 // Fields of the movers of one chunk of 8 slots, copied to the staging area in
 // one batch of independent loads (stg: chunk-local mover bits).
 __device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint32_t stg,
-                                  int nargs, const ArgPtrs& args, const MoveScratch& ms) {
+                                  int a_first, int nargs, const ArgPtrs& args,
+                                  const MoveScratch& ms) {
     if (!stg) return;
-    for (int a = 0; a < nargs; ++a) {
+    for (int a = a_first; a < nargs; ++a) {
         const double* __restrict__ src = args.p[a];
         double* __restrict__ dst = ms.stga + a * S;
         double v[8];
@@ -134,17 +135,27 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
     uint64_t mov = 0;
     double* __restrict__ sc = static_cast<double*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
+    const double* __restrict__ a0 = nargs > 0 ? args.p[0] : nullptr;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
-    // loads and stores across neighbouring cells)
+    // loads and stores across neighbouring cells); the coordinates and the
+    // first field of a chunk's particles are loaded together, a chunk ahead
     constexpr int B = 8;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
+    struct Buf {
         double pb[B][D];
+        double fa[B];
+    };
+    auto load = [&](int k0, Buf& b) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (occ >> (k0 + j)) & 1;
+            const int64_t e = c + C * (k0 + j);
 #pragma unroll
-            for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + C * (k0 + j)] : 0.0;
+            for (int d = 0; d < D; ++d) b.pb[j][d] = on ? coords.p[d][e] : 0.0;
+            b.fa[j] = (on && a0) ? a0[e] : 0.0;
         }
+    };
+    auto proc = [&](int k0, const Buf& b) __attribute__((always_inline)) {
+        const auto& pb = b.pb;
         uint64_t word = 0;
         uint32_t mv8 = 0, stg = 0;
 #pragma unroll
@@ -193,6 +204,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     if (n != kCenter<D> && !inJ) cd |= kGap;
 #pragma unroll
                     for (int d = 0; d < D; ++d) sc[d * S + c + C * k] = p[d];
+                    if (a0) ms.stga[c + C * k] = b.fa[j];
                     stg |= 1u << j;
                 }
             }
@@ -200,8 +212,9 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args(c, C, S, k0, stg, nargs, args, ms);
-    }
+        stage_args(c, C, S, k0, stg, 1, nargs, args, ms);
+    };
+    walk_chunks<Buf>(occ, g.K, load, proc);
     ms.occ0[c] = occ;
     ms.occ[c] = occ;
     ms.mov[c] = mov;
@@ -227,7 +240,7 @@ template <int D, int BS>
 __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3 off, int phase,
                                                  const int* __restrict__ flag, jp_stats* st) {
     __shared__ uint64_t s_occ[kNB<D>][BS];
-    if (*flag) return;
+    const int stop = *flag;  // checked below, after this cell's loads are issued
     int64_t ncol[D];
     int64_t total = 1;
 #pragma unroll
@@ -251,11 +264,19 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     const int K = g.K;
     const int64_t C = g.C;
     const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
-    // everything this cell needs, in one round of independent loads
+    // everything this cell needs, in one round of independent loads: its
+    // mover/arrival words and code words, and occupancy words of its 3^D
+    // neighbourhood; in 2D also the neighbours' arrival and touched words.
+    // Cells of one colour are 3 apart, so no other thread of this phase reads
+    // or writes these neighbourhood words and the values loaded now are still
+    // current at the end (loading them late waits for this thread's earlier
+    // stores).  In 3D the 27 extra pairs would cost the occupancy they save.
+    constexpr bool kPre = D == 2;
     uint64_t pend = ms.mov[S];
     uint64_t arrS = ms.arr[S];
+    const uint64_t tchS = ms.tch[S];
     uint64_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // movers' code bytes, 8 per word
-    if (pend) {
+    {
         const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S;
         const int nw = (K + 7) >> 3;
 #pragma unroll
@@ -263,15 +284,19 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
             if (j < nw) cw[j] = cp[j * C];
     }
     int64_t nbc[kNB<D>];
-    uint64_t was[kNB<D>];
+    uint64_t was[kNB<D>], arrN[kNB<D>], tchN[kNB<D>];
 #pragma unroll
     for (int n = 0; n < kNB<D>; ++n) {
         bool v;
         nbc[n] = nb_cell<D>(g, I, n, v);
+        const bool side = kPre && v && n != kCenter<D>;
         was[n] = v ? ms.occ[nbc[n]] : ~0ull;  // outside the grid: never free
-        s_occ[n][tid] = was[n];
+        arrN[n] = side ? ms.arr[nbc[n]] : 0ull;
+        tchN[n] = side ? ms.tch[nbc[n]] : 0ull;
     }
-    if (!pend) return;  // nothing leaves or re-slots in this cell
+    if (stop || !pend) return;  // serial fallback, or nothing leaves or re-slots here
+#pragma unroll
+    for (int n = 0; n < kNB<D>; ++n) s_occ[n][tid] = was[n];
     uint64_t selfm = 0;  // slots that received a re-slotted particle of this cell
     uint64_t selfins = 0;
     int sp = 0;
@@ -341,13 +366,18 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
         if (now != was[n]) {
             const uint64_t ins = now & ~was[n];
             ms.occ[nbc[n]] = now;
-            ms.arr[nbc[n]] |= ins;
-            ms.tch[nbc[n]] |= ins;
+            if constexpr (kPre) {
+                ms.arr[nbc[n]] = arrN[n] | ins;
+                ms.tch[nbc[n]] = tchN[n] | ins;
+            } else {
+                ms.arr[nbc[n]] |= ins;
+                ms.tch[nbc[n]] |= ins;
+            }
         }
     }
     ms.occ[S] = s_occ[kCenter<D>][tid];
     ms.arr[S] = arrS;
-    ms.tch[S] |= selfins;
+    ms.tch[S] = tchS | selfins;
     ms.mov[S] = 0;
     stat_add(st, ST_MOVED, n_moved);
     stat_add(st, ST_DELETED, n_del);
@@ -590,15 +620,24 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
     uint64_t mov = 0;
     float* __restrict__ sc = static_cast<float*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
+    const double* __restrict__ a0 = nargs > 0 ? args.p[0] : nullptr;
     constexpr int B = 8;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
+    struct Buf {
         float ub[B][D];
+        double fa[B];
+    };
+    auto load = [&](int k0, Buf& b) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (occ >> (k0 + j)) & 1;
+            const int64_t e = c + C * (k0 + j);
 #pragma unroll
-            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + C * (k0 + j)] : 0.0f;
+            for (int d = 0; d < D; ++d) b.ub[j][d] = on ? off.p[d][e] : 0.0f;
+            b.fa[j] = (on && a0) ? a0[e] : 0.0;
         }
+    };
+    auto proc = [&](int k0, const Buf& b) __attribute__((always_inline)) {
+        const auto& ub = b.ub;
         uint64_t word = 0;
         uint32_t mv8 = 0, stg = 0;
 #pragma unroll
@@ -633,14 +672,16 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
 #pragma unroll
                 for (int d = 0; d < D; ++d)
                     sc[d * S + c + C * k] = (float)((double)ub[j][d] - (double)s[d]);
+                if (a0) ms.stga[c + C * k] = b.fa[j];
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args(c, C, S, k0, stg, nargs, args, ms);
-    }
+        stage_args(c, C, S, k0, stg, 1, nargs, args, ms);
+    };
+    walk_chunks<Buf>(occ, g.K, load, proc);
     ms.occ0[c] = occ;
     ms.occ[c] = occ;
     ms.mov[c] = mov;
