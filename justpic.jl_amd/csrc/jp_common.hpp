@@ -224,6 +224,49 @@ __device__ inline void walk_chunks(uint64_t bits, int K, Load&& load, Proc&& pro
     }
 }
 
+// Nearest-neighbour search of the injection on squared distances.
+// wave_argmin: lexicographic minimum of (s, order) over the wavefront.
+__device__ inline void wave_argmin(double& best, int& border) {
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) {
+        const double ob = __shfl_xor(best, sh, 64);
+        const int oo = __shfl_xor(border, sh, 64);
+        if (ob < best || (ob == best && oo < border)) {
+            best = ob;
+            border = oo;
+        }
+    }
+}
+
+// The reference takes the first element (in order) of minimal distance
+// sqrt(s), with sqrt correctly rounded.  sqrt is monotonic, so the minimal
+// distance is sqrt(best); an element with s > best has the same rounded sqrt
+// only if s <= best (1 + 2^-51) (half an ulp either side), so elements above
+// best (1 + 2^-50) never tie and the others are checked with sqrt itself.
+// s9[nb] = squared distance of this lane's element of neighbour nb (inf when
+// absent); order = nb * 64 + lane.
+template <int NB>
+__device__ inline void resolve_ties(const double (&s9)[NB], double best, int& border, int lane) {
+    const double lim = best * (1.0 + 0x1p-50);
+    bool near = false;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) near = near || (s9[nb] > best && s9[nb] <= lim);
+    if (!__any(near)) return;
+    const double m = sqrt(best);
+    int o = 0x7fffffff;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+        if (s9[nb] > best && s9[nb] <= lim && sqrt(s9[nb]) == m) o = min(o, nb * 64 + lane);
+#pragma unroll
+    for (int sh = 32; sh >= 1; sh >>= 1) o = min(o, __shfl_xor(o, sh, 64));
+    border = min(border, o);
+}
+
+// a neighbour cell whose squared bound exceeds best (1 + 2^-40) cannot hold
+// an element within the tie range above (the margin covers the rounding of
+// the bound)
+constexpr double kPrune = 1.0 + 0x1p-40;
+
 template <int NQ>
 __device__ inline void wave_segment(bool want, int col, int32_t c, int64_t wave, int64_t nwave,
                                     int32_t* __restrict__ seg, int32_t* __restrict__ cnt) {

@@ -561,31 +561,30 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                 // of the minimum in (cell, slot) order
                 double best;
                 int border;
+                // best holds the smallest SQUARED distance: the reference's
+                // argmin of sqrt(s) with strict < (first in order on ties) is
+                // the argmin of s, except that a larger s can round to the same
+                // sqrt and come first; such near ties are resolved exactly
                 auto search = [&]() __attribute__((always_inline)) {
                     best = __builtin_inf();
                     border = 0x7fffffff;
+                    double s9[NB];
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) {
+                        s9[nb] = __builtin_inf();
                         if (!((loaded >> nb) & 1)) continue;
                         const bool av = nb == CENTER ? (act && lane != ip) : ((o9[nb] >> lane) & 1);
                         if (!av) continue;
                         if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
                         const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
-                        const double dd = sqrt(du * du + dv * dv);
-                        if (dd < best) {
-                            best = dd;
+                        s9[nb] = du * du + dv * dv;
+                        if (s9[nb] < best) {
+                            best = s9[nb];
                             border = nb * 64 + lane;
                         }
                     }
-#pragma unroll
-                    for (int s = 32; s >= 1; s >>= 1) {
-                        const double ob = __shfl_xor(best, s, 64);
-                        const int oo = __shfl_xor(border, s, 64);
-                        if (ob < best || (ob == best && oo < border)) {
-                            best = ob;
-                            border = oo;
-                        }
-                    }
+                    wave_argmin(best, border);
+                    resolve_ties(s9, best, border, lane);
                 };
                 search();
                 // neighbour cells whose box is not farther than the best so far
@@ -597,7 +596,7 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords,
                                                         : (pn[0] > bh[nb][0] ? pn[0] - bh[nb][0] : 0.0);
                     const double gy = bl[nb][1] > pn[1] ? bl[nb][1] - pn[1]
                                                         : (pn[1] > bh[nb][1] ? pn[1] - bh[nb][1] : 0.0);
-                    if (!(sqrt(gx * gx + gy * gy) > best)) need |= 1u << nb;
+                    if (!(gx * gx + gy * gy > best * kPrune)) need |= 1u << nb;
                 }
                 if (need) {
                     // one predicate per lane (no uniform branch per cell), so the

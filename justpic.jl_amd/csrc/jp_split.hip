@@ -823,11 +823,15 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                 }
                 double best;
                 int border;
+                // best holds the smallest SQUARED distance; near ties of the
+                // rounded distance are resolved exactly (as in the FP64 kernel)
                 auto search = [&]() __attribute__((always_inline)) {
                     best = __builtin_inf();
                     border = 0x7fffffff;
+                    double s9[NB];
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) {
+                        s9[nb] = __builtin_inf();
                         if (!((loaded >> nb) & 1)) continue;
                         const bool av = nb == CENTER ? (act && lane != ip) : ((o9[nb] >> lane) & 1);
                         if (!av) continue;
@@ -835,21 +839,14 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                             ((double)(nb % 3 - 1) + (double)uq[nb][0] - (double)un[0]) * h0;
                         const double r1 =
                             ((double)(nb / 3 - 1) + (double)uq[nb][1] - (double)un[1]) * h1;
-                        const double dd = sqrt(r0 * r0 + r1 * r1);
-                        if (dd < best) {
-                            best = dd;
+                        s9[nb] = r0 * r0 + r1 * r1;
+                        if (s9[nb] < best) {
+                            best = s9[nb];
                             border = nb * 64 + lane;
                         }
                     }
-#pragma unroll
-                    for (int s = 32; s >= 1; s >>= 1) {
-                        const double ob = __shfl_xor(best, s, 64);
-                        const int oo = __shfl_xor(border, s, 64);
-                        if (ob < best || (ob == best && oo < border)) {
-                            best = ob;
-                            border = oo;
-                        }
-                    }
+                    wave_argmin(best, border);
+                    resolve_ties(s9, best, border, lane);
                 };
                 search();
                 uint32_t need = 0;
@@ -864,7 +861,7 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
                     const double hi1 = ((double)(nb / 3 - 1) + (double)bh[nb][1] - (double)un[1]) * h1;
                     const double gx = lo0 > 0.0 ? lo0 : (hi0 < 0.0 ? hi0 : 0.0);
                     const double gy = lo1 > 0.0 ? lo1 : (hi1 < 0.0 ? hi1 : 0.0);
-                    if (!(sqrt(gx * gx + gy * gy) > best)) need |= 1u << nb;
+                    if (!(gx * gx + gy * gy > best * kPrune)) need |= 1u << nb;
                 }
                 if (need) {
                     // one predicate per lane (no uniform branch per cell), so the
