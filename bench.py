@@ -233,7 +233,8 @@ def run_ensemble(jp, args):
     move_particles! and particle2grid!, captured once in a hipGraph."""
     ctx = jp.Context(0, synchronous=False)
     log("building ensemble of", args.members)
-    ens, fields, nodal = jp.random_ensemble(ctx, args.members, seed=args.seed)
+    ens, fields, nodal = jp.random_ensemble(ctx, args.members, seed=args.seed,
+                                            streams=args.streams)
     ctx.sync()
     best, ms = ens.tune(reps=3)
     log("tuned variant", best, "ms per variant", [round(t, 3) for t in ms])
@@ -288,6 +289,7 @@ def run_ensemble(jp, args):
                                f"{{64,128,192,256,384,512}}^2, README stream function at "
                                f"amplitude U[50,400], own CFL dt, nxcell 24, max_xcell 48",
                    "members": args.members, "cells": cells, "particles": int(P),
+                   "streams": args.streams,
                    "tuned_variant": best, "variant_ms": ms},
         "roofline": {"bound": "hbm", "kernel": "ensemble advection (persistent, tuned)",
                      "achieved": adv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -435,6 +437,8 @@ def main():
     ap.add_argument("--descent", type=int, default=3,
                     help="steepest-descent steps of the inverse problem (--case adjoint)")
     ap.add_argument("--members", type=int, default=64, help="ensemble size (--case ensemble)")
+    ap.add_argument("--streams", type=int, default=8,
+                    help="streams the ensemble's per-grid calls are spread over")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")

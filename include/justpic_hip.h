@@ -104,6 +104,10 @@ int jp_fill_f32(jp_ctx* ctx, float* dptr, float value, int64_t count);
 int jp_event_create(jp_ctx* ctx, void** ev);
 int jp_event_destroy(jp_ctx* ctx, void* ev);
 int jp_event_record(jp_ctx* ctx, void* ev);
+/* the context stream waits for an event recorded on another context's stream
+ * (fork/join of independent work; inside a capture it joins the other stream
+ * into the graph) */
+int jp_event_wait(jp_ctx* ctx, void* ev);
 int jp_event_elapsed_ms(jp_ctx* ctx, void* ev_start, void* ev_stop, float* ms);
 
 /* hipGraph capture of whatever entry points are called between begin and end

@@ -321,6 +321,12 @@ int jp_event_record(jp_ctx* ctx, void* ev) {
     return JP_OK;
 }
 
+int jp_event_wait(jp_ctx* ctx, void* ev) {
+    if (!ctx || !ev) return JP_ERR_ARG;
+    JP_HIP(ctx, hipStreamWaitEvent(ctx->stream, static_cast<hipEvent_t>(ev), 0));
+    return JP_OK;
+}
+
 int jp_event_elapsed_ms(jp_ctx* ctx, void* a, void* b, float* ms) {
     if (!ctx || !a || !b || !ms) return JP_ERR_ARG;
     JP_HIP(ctx, hipEventSynchronize(static_cast<hipEvent_t>(b)));

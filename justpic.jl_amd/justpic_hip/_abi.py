@@ -93,6 +93,7 @@ _SIGS = {
     "jp_event_create": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "jp_event_destroy": (C.c_int, [_P, _P]),
     "jp_event_record": (C.c_int, [_P, _P]),
+    "jp_event_wait": (C.c_int, [_P, _P]),
     "jp_event_elapsed_ms": (C.c_int, [_P, _P, _P, C.POINTER(C.c_float)]),
     "jp_graph_begin": (C.c_int, [_P]),
     "jp_graph_end": (C.c_int, [_P, C.POINTER(C.c_void_p)]),

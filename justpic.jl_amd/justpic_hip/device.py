@@ -68,6 +68,10 @@ class Context:
     def record(self, ev):
         self.check(self.lib.jp_event_record(self.h, ev), "event_record")
 
+    def wait(self, ev):
+        """This context's stream waits for `ev` (recorded on any context)."""
+        self.check(self.lib.jp_event_wait(self.h, ev), "event_wait")
+
     def elapsed_ms(self, a, b) -> float:
         ms = C.c_float()
         self.check(self.lib.jp_event_elapsed_ms(self.h, a, b, C.byref(ms)), "elapsed")

@@ -6,6 +6,12 @@ persistent launch (``jp_ensemble_advect``); a full time step (advection, then
 per grid ``move_particles!`` with its fields and ``particle2grid!``) can be
 captured once into a hipGraph and replayed.  :meth:`Ensemble.tune` times the
 launch variants on the ensemble itself and keeps the fastest.
+
+The per-grid calls of a step are independent, and most grids are too small to
+fill the GPU on their own, so they are spread over a few asynchronous
+sub-contexts (each its own stream and scratch; largest grid first onto the
+least-loaded stream), forked after the advection with an event and joined
+before the step returns.  A captured step records the same fork and join.
 """
 from __future__ import annotations
 
@@ -14,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from .api import RungeKutta2, move_particles_, particle2grid_
+from .api import RungeKutta2
 from .device import Context, DeviceArray
 
 
@@ -23,7 +29,7 @@ class Ensemble:
     container, its two velocity components (DeviceArrays on the staggered
     grids) and its time step.  The arrays must outlive the ensemble."""
 
-    def __init__(self, ctx: Context, members, method=None):
+    def __init__(self, ctx: Context, members, method=None, streams: int = 8):
         method = method or RungeKutta2()
         self.ctx = ctx
         self.members = list(members)
@@ -47,6 +53,18 @@ class Ensemble:
                   "ensemble")
         self.h = h
         self._keep = (grids, vgs, off, idx, vel, dts)
+        # sub-contexts for the per-grid calls, members balanced by slot count
+        ns = max(1, min(int(streams), n))
+        self.subs = [Context(ctx.device, synchronous=False) for _ in range(ns)]
+        load = [0] * ns
+        self.lane = [0] * n
+        for i in sorted(range(n), key=lambda i: -self.members[i][0].C * self.members[i][0].max_xcell):
+            j = min(range(ns), key=lambda j: load[j])
+            self.lane[i] = j
+            p = self.members[i][0]
+            load[j] += p.C * p.max_xcell
+        self._fork = ctx.event()
+        self._join = [sc.event() for sc in self.subs]
 
     @property
     def n(self):
@@ -74,18 +92,35 @@ class Ensemble:
 
     def step(self, fields, nodal):
         """advection! for all members, then per member move_particles!(p,
-        (field,)) and particle2grid!(F, field, p)."""
+        (field,)) and particle2grid!(F, field, p), the members' calls on the
+        sub-context streams."""
+        self._step(fields, nodal, sync=self.ctx.synchronous)
+
+    def _step(self, fields, nodal, sync):
+        ctx, lib = self.ctx, self.ctx.lib
         self.advect()
-        for (p, _, _), f, F in zip(self.members, fields, nodal):
-            move_particles_(p, (f,))
-            particle2grid_(F, f, p)
+        ctx.record(self._fork)
+        for sc in self.subs:
+            sc.wait(self._fork)
+        for i, ((p, _, _), f, F) in enumerate(zip(self.members, fields, nodal)):
+            sc = self.subs[self.lane[i]]
+            sc.check(lib.jp_move_particles_split(sc.h, p.grid_ref(), p.coord_ptrs(), p.index.ptr,
+                                                 1, _abi.ptr_array([f.ptr]), None),
+                     "move_particles!")
+            sc.check(lib.jp_particle2grid_split(sc.h, p.grid_ref(), F.ptr, f.ptr, p.coord_ptrs(),
+                                                p.index.ptr), "particle2grid!")
+        for sc, ev in zip(self.subs, self._join):
+            sc.record(ev)
+            ctx.wait(ev)
+        if sync:
+            ctx.sync()
 
     def capture_step(self, fields, nodal):
         """hipGraph of one :meth:`step` (replay with ``ctx.graph_launch``).
         Run one eager step first so that the context's scratch is sized."""
         self.ctx.graph_begin()
         try:
-            self.step(fields, nodal)
+            self._step(fields, nodal, sync=False)
         finally:
             g = self.ctx.graph_end()
         return g
@@ -94,6 +129,14 @@ class Ensemble:
         if getattr(self, "h", None) and self.h.value:
             self.ctx.lib.jp_ensemble_destroy(self.ctx.h, self.h)
             self.h = C.c_void_p()
+        for sc, ev in zip(getattr(self, "subs", ()), getattr(self, "_join", ())):
+            sc.sync()
+            sc.lib.jp_event_destroy(sc.h, ev)
+            sc.close()
+        if getattr(self, "_fork", None) is not None:
+            self.ctx.lib.jp_event_destroy(self.ctx.h, self._fork)
+            self._fork = None
+        self.subs, self._join = [], []
 
     def __del__(self):
         try:
@@ -104,7 +147,7 @@ class Ensemble:
 
 def random_ensemble(ctx: Context, n: int, seed: int = 0,
                     sizes=(64, 128, 192, 256, 384, 512), amp=(50.0, 400.0),
-                    nxcell=24, max_xcell=48, min_xcell=12):
+                    nxcell=24, max_xcell=48, min_xcell=12, streams: int = 8):
     """The ensemble workload: per member a cell count drawn from ``sizes``
     (squared), the README stream function at an amplitude drawn uniform in
     ``amp``, its own CFL time step, seeded positions in the FP32-split form.
@@ -127,4 +170,4 @@ def random_ensemble(ctx: Context, n: int, seed: int = 0,
         members.append((p, V, cfg["dt"]))
         fields.append(f)
         nodal.append(nodal_field(p))
-    return Ensemble(ctx, members), fields, nodal
+    return Ensemble(ctx, members, streams=streams), fields, nodal

@@ -117,3 +117,32 @@ def test_ensemble_rejects_fp64_members(ctx):
     q = jp.to_f64(p)
     with pytest.raises(ValueError):
         jp.Ensemble(ctx, [(q, V, dt)])
+
+
+def test_ensemble_step_matches_per_grid_calls(ctx):
+    """The step fans the members' calls out over sub-context streams; every
+    member must end exactly as with plain per-grid calls on one context."""
+    say("fan-out vs per-grid calls")
+    a = build(ctx, seed=13)
+    b = build(ctx, seed=13)
+    ea = jp.Ensemble(ctx, [(p, V, dt) for p, V, dt, _ in a], streams=3)
+    eb = jp.Ensemble(ctx, [(p, V, dt) for p, V, dt, _ in b])
+    Fa = [jp.nodal_field(p) for p, *_ in a]
+    Fb = [jp.nodal_field(p) for p, *_ in b]
+    for _ in range(3):
+        ea.step([f for *_, f in a], Fa)
+        eb.advect()
+        for (p, _, _, f), F in zip(b, Fb):
+            jp.move_particles_(p, (f,))
+            jp.particle2grid_(F, f, p)
+    ctx.sync()
+    for (pa, *_, x), (pb, *_, y), X, Y in zip(a, b, Fa, Fb):
+        oa, ia = state(pa)
+        ob, ib = state(pb)
+        assert np.array_equal(ia, ib)
+        for d in range(2):
+            assert np.array_equal(oa[d], ob[d], equal_nan=True)
+        assert np.array_equal(x.data.flat(), y.data.flat(), equal_nan=True)
+        assert np.array_equal(X.flat(), Y.flat(), equal_nan=True)
+    ea.close()
+    eb.close()
