@@ -23,8 +23,11 @@
 //    to it bit for bit.
 //  - k_adj_bwd: blocks of steps in reverse order; a block is recomputed from
 //    its checkpoint into a per-thread LDS tape (the position at the start of
-//    each step), then reversed step by step.  The velocity cotangent goes to
-//    Vbar with FP64 atomic adds; x0bar and Tpbar are per-thread values.
+//    each step), then reversed step by step.  x0bar and Tpbar are per-thread
+//    values.  In 2D the velocity cotangent of a workgroup and block of steps
+//    is accumulated in an LDS tile covering exactly the nodes the recompute
+//    touched (LDS atomic adds), then added to Vbar once; when that footprint
+//    exceeds the tile, and in 3D, it goes to Vbar with FP64 atomic adds.
 // The adjoint is the exact derivative of the forward model on the branch the
 // forward pass takes (brackets, cells and survivors fixed): the n-linear
 // interpolation's weights and slopes, the integrator's stage weights, and for
@@ -81,10 +84,42 @@ struct Corners {
     double v[1 << D];
     double t[D];
     double iv[D];
+    int64_t k[D];  // bracket (lower corner node) per dimension
 };
 
+// bracket observers of gather: NoTrack, or BoxTrack (per component and
+// dimension, the range of lower-corner nodes seen)
+struct NoTrack {
+    template <int D>
+    __device__ void operator()(int, const int64_t (&)[D]) const {}
+};
 template <int D>
-__device__ inline void gather(const AdjArgs<D>& A, int c, const double (&p)[D], Corners<D>& K) {
+struct BoxTrack {
+    int lo[D][D], hi[D][D];
+    __device__ BoxTrack() {
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                lo[c][d] = 0x7fffffff;
+                hi[c][d] = -1;
+            }
+    }
+    __device__ void operator()(int c, const int64_t (&k)[D]) {
+#pragma unroll
+        for (int cc = 0; cc < D; ++cc)
+            if (cc == c)
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    lo[cc][d] = min(lo[cc][d], (int)k[d]);
+                    hi[cc][d] = max(hi[cc][d], (int)k[d]);
+                }
+    }
+};
+
+template <int D, class Tr>
+__device__ inline void gather(const AdjArgs<D>& A, int c, const double (&p)[D], Corners<D>& K,
+                              Tr& tr) {
     int64_t k[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -92,7 +127,9 @@ __device__ inline void gather(const AdjArgs<D>& A, int c, const double (&p)[D], 
         k[d] = adj_bracket(p[d], X);
         K.iv[d] = X.idx[k[d]];
         K.t[d] = (p[d] - X.x[k[d]]) * K.iv[d];
+        K.k[d] = k[d];
     }
+    tr(c, k);
     const int64_t n0 = A.vg.ax[c][0].n, n1 = A.vg.ax[c][1].n;
     const double* __restrict__ Vc = A.V[c];
 #pragma unroll
@@ -120,29 +157,47 @@ __device__ inline double lerp_n(const Corners<D>& K) {
 
 // velocity component c at p; +Inf outside its grid (check_local_limits,
 // src/Advection/advection.jl:83-91)
-template <int D>
-__device__ inline double vel_c(const AdjArgs<D>& A, int c, const double (&p)[D]) {
+template <int D, class Tr>
+__device__ inline double vel_c(const AdjArgs<D>& A, int c, const double (&p)[D], Tr& tr) {
     if (!in_limits(A.vg, c, p)) return __builtin_inf();
     Corners<D> K;
-    gather(A, c, p, K);
+    gather(A, c, p, K, tr);
     return lerp_n(K);
+}
+
+template <int D, class Tr>
+__device__ inline void vel(const AdjArgs<D>& A, const double (&p)[D], double (&v)[D], Tr& tr) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) v[c] = vel_c(A, c, p, tr);
 }
 
 template <int D>
 __device__ inline void vel(const AdjArgs<D>& A, const double (&p)[D], double (&v)[D]) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) v[c] = vel_c(A, c, p);
+    NoTrack tr;
+    vel(A, p, v, tr);
 }
+
+// destination of velocity cotangents: FP64 atomic adds to Vbar in global
+// memory, or into the workgroup's LDS tile (component c's nodes lo[c] ..
+// lo[c] + ext[c] - 1 per dimension, at tile offset off[c]) that is added to
+// Vbar once per block of steps
+template <int D>
+struct VSink {
+    double* g[D];
+    double* tile;  // nullptr: global
+    int lo[D][D], ext[D][D], off[D];
+};
 
 // adjoint of component c at p for the cotangent vb: Vbar[c] += weights*vb,
 // pb += slope*vb.  Only called on points the forward pass found inside the
 // component's grid.
 template <int D>
 __device__ inline void vel_c_bw(const AdjArgs<D>& A, int c, const double (&p)[D], double vb,
-                                double* __restrict__ Vbc, double (&pb)[D]) {
+                                const VSink<D>& S, double (&pb)[D]) {
     if (vb == 0.0) return;
     Corners<D> K;
-    gather(A, c, p, K);
+    NoTrack nt;
+    gather(A, c, p, K, nt);
     const double* v = K.v;
     const double* t = K.t;
     double dd[D];
@@ -170,20 +225,35 @@ __device__ inline void vel_c_bw(const AdjArgs<D>& A, int c, const double (&p)[D]
     }
 #pragma unroll
     for (int d = 0; d < D; ++d) pb[d] += dd[d] * K.iv[d] * vb;
+    int base = 0;  // tile index of the lower corner
+    if (S.tile) {
+        int stride = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            base += ((int)K.k[d] - S.lo[c][d]) * stride;
+            stride *= S.ext[c][d];
+        }
+    }
 #pragma unroll
     for (int q = 0; q < (1 << D); ++q) {
         double w = 1.0;
 #pragma unroll
         for (int d = 0; d < D; ++d) w *= ((q >> d) & 1) ? t[d] : 1.0 - t[d];
-        unsafeAtomicAdd(Vbc + K.at[q], w * vb);
+        if (S.tile) {
+            int o = base + (q & 1) + ((q >> 1) & 1) * S.ext[c][0];
+            if constexpr (D == 3) o += ((q >> 2) & 1) * S.ext[c][0] * S.ext[c][1];
+            unsafeAtomicAdd(S.tile + S.off[c] + o, w * vb);
+        } else {
+            unsafeAtomicAdd(S.g[c] + K.at[q], w * vb);
+        }
     }
 }
 
 // one step (Euler.jl, RK2.jl with @muladd, RK4.jl unfused)
-template <int D, int M>
-__device__ inline void step(const AdjArgs<D>& A, const double (&p0)[D], double (&out)[D]) {
+template <int D, int M, class Tr>
+__device__ inline void step(const AdjArgs<D>& A, const double (&p0)[D], double (&out)[D], Tr& tr) {
     double v0[D];
-    vel(A, p0, v0);
+    vel(A, p0, v0, tr);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = fma(A.dt, v0[d], p0[d]);
@@ -191,7 +261,7 @@ __device__ inline void step(const AdjArgs<D>& A, const double (&p0)[D], double (
         double p1[D], v1[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) p1[d] = fma(A.adt, v0[d], p0[d]);
-        vel(A, p1, v1);
+        vel(A, p1, v1, tr);
         if (A.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = fma(A.dt, v1[d], p0[d]);
@@ -203,13 +273,13 @@ __device__ inline void step(const AdjArgs<D>& A, const double (&p0)[D], double (
         double q[D], k2[D], k3[D], k4[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + (A.dt * v0[d]) / 2.0;
-        vel(A, q, k2);
+        vel(A, q, k2, tr);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + (A.dt * k2[d]) / 2.0;
-        vel(A, q, k3);
+        vel(A, q, k3, tr);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + A.dt * k3[d];
-        vel(A, q, k4);
+        vel(A, q, k4, tr);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const double sum = ((v0[d] + 2.0 * k2[d]) + 2.0 * k3[d]) + k4[d];
@@ -218,16 +288,22 @@ __device__ inline void step(const AdjArgs<D>& A, const double (&p0)[D], double (
     }
 }
 
+template <int D, int M>
+__device__ inline void step(const AdjArgs<D>& A, const double (&p0)[D], double (&out)[D]) {
+    NoTrack tr;
+    step<D, M>(A, p0, out, tr);
+}
+
 // reverse of step at p0: pb is the output's cotangent on entry, p0's on exit
 template <int D, int M>
 __device__ inline void step_bw(const AdjArgs<D>& A, const double (&p0)[D], double (&pb)[D],
-                               const Ptr3& Vb) {
+                               const VSink<D>& Vb) {
     double ob[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) ob[d] = pb[d];
     if constexpr (M == JP_EULER) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p0, A.dt * ob[c], Vb.p[c], pb);
+        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p0, A.dt * ob[c], Vb, pb);
     } else if constexpr (M == JP_RK2) {
         double v0[D], p1[D], v0b[D], v1b[D], p1b[D];
         vel(A, p0, v0);
@@ -249,14 +325,14 @@ __device__ inline void step_bw(const AdjArgs<D>& A, const double (&p0)[D], doubl
 #pragma unroll
         for (int d = 0; d < D; ++d) p1b[d] = 0.0;
 #pragma unroll
-        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p1, v1b[c], Vb.p[c], p1b);
+        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p1, v1b[c], Vb, p1b);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             pb[d] += p1b[d];
             v0b[d] += A.adt * p1b[d];
         }
 #pragma unroll
-        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p0, v0b[c], Vb.p[c], pb);
+        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p0, v0b[c], Vb, pb);
     } else {
         double k1[D], k2[D], k3[D], q2[D], q3[D], q4[D], kb[4][D], qb[D];
         vel(A, p0, k1);
@@ -282,7 +358,7 @@ __device__ inline void step_bw(const AdjArgs<D>& A, const double (&p0)[D], doubl
 #pragma unroll
             for (int d = 0; d < D; ++d) qb[d] = 0.0;
 #pragma unroll
-            for (int c = 0; c < D; ++c) vel_c_bw(A, c, qs, kb[st][c], Vb.p[c], qb);
+            for (int c = 0; c < D; ++c) vel_c_bw(A, c, qs, kb[st][c], Vb, qb);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 pb[d] += qb[d];
@@ -290,7 +366,7 @@ __device__ inline void step_bw(const AdjArgs<D>& A, const double (&p0)[D], doubl
             }
         }
 #pragma unroll
-        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p0, kb[0][c], Vb.p[c], pb);
+        for (int c = 0; c < D; ++c) vel_c_bw(A, c, p0, kb[0][c], Vb, pb);
     }
 }
 
@@ -510,24 +586,47 @@ __global__ __launch_bounds__(kBlock) void k_adj_zero(double* __restrict__ p, int
     if (i < n) p[i] = 0.0;
 }
 
+// LDS tile for the velocity cotangents of one workgroup and block of steps
+// (2D only): whatever the 64 KB workgroup budget leaves after the tape.  A
+// workgroup of 128 particles from a compact patch touches ~2-4 k nodes over
+// ten steps; in 3D the footprint rarely fits and global atomics are used.
+constexpr size_t kLdsBudget = 65536 - 256;  // bytes, minus the static box words
+template <int D>
+inline int tile_cap(int every) {
+    if (D != 2) return 0;
+    const size_t tape = (size_t)every * D * kAdjBlock * sizeof(double);
+    return tape < kLdsBudget ? (int)((kLdsBudget - tape) / sizeof(double)) : 0;
+}
+
 template <int D, int M>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_bwd(
     AdjArgs<D> A, const double* __restrict__ ckpt, const uint8_t* __restrict__ alive,
     const double* __restrict__ Tp, const double* __restrict__ F, const double* __restrict__ W,
-    const double* __restrict__ Fbar, Ptr3 Vb, Ptr3 x0b, double* __restrict__ Tpb) {
-    extern __shared__ double tape[];  // [every][D][kAdjBlock]: step-start positions
+    const double* __restrict__ Fbar, Ptr3 Vb, Ptr3 x0b, double* __restrict__ Tpb, int cap) {
+    // dynamic LDS: the tape [every][D][kAdjBlock] of step-start positions,
+    // then the cotangent tile of cap doubles
+    extern __shared__ double smem[];
+    double* tape = smem;
+    double* tile = smem + A.every * D * kAdjBlock;
+    __shared__ int sbox[D][D][2];
     const int lt = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kAdjBlock + lt;
-    if (i >= A.np) return;  // no block-wide synchronisation below
+    // every thread reaches the block-wide barriers below
+    const bool live = i < A.np && alive[i];
+    VSink<D> S;
+#pragma unroll
+    for (int c = 0; c < D; ++c) S.g[c] = Vb.p[c];
+    S.tile = nullptr;
     double pb[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) pb[d] = 0.0;
     double tb = 0.0;
-    if (alive[i]) {
-        const int nblk = A.nsteps == 0 ? 1 : (A.nsteps + A.every - 1) / A.every;
-        for (int b = nblk - 1; b >= 0; --b) {
-            const int s0 = b * A.every;
-            const int n = A.nsteps - s0 < A.every ? A.nsteps - s0 : A.every;
+    const int nblk = A.nsteps == 0 ? 1 : (A.nsteps + A.every - 1) / A.every;
+    for (int b = nblk - 1; b >= 0; --b) {
+        const int s0 = b * A.every;
+        const int n = A.nsteps - s0 < A.every ? A.nsteps - s0 : A.every;
+        BoxTrack<D> box;
+        if (live) {
             double x[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) x[d] = ckpt[((int64_t)b * D + d) * A.np + i];
@@ -535,7 +634,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_bwd(
 #pragma unroll
                 for (int d = 0; d < D; ++d) tape[(j * D + d) * kAdjBlock + lt] = x[d];
                 double y[D];
-                step<D, M>(A, x, y);
+                step<D, M>(A, x, y, box);
 #pragma unroll
                 for (int d = 0; d < D; ++d) x[d] = y[d];
             }
@@ -557,17 +656,86 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_bwd(
                     for (int d = 0; d < D; ++d) pb[d] += gw * (-2.0 * (x[d] - xn[d]) * wi * wi);
                 }
             }
+        }
+        int total = 0;
+        if constexpr (D == 2) {
+            // the workgroup's node range per component and dimension
+            if (lt < D * D) {
+                (&sbox[0][0][0])[2 * lt] = 0x7fffffff;
+                (&sbox[0][0][0])[2 * lt + 1] = -1;
+            }
+            __syncthreads();
+            if (live && n > 0) {
+#pragma unroll
+                for (int c = 0; c < D; ++c)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        atomicMin(&sbox[c][d][0], box.lo[c][d]);
+                        atomicMax(&sbox[c][d][1], box.hi[c][d]);
+                    }
+            }
+            __syncthreads();
+            bool any = true;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                int sz = 1;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    S.lo[c][d] = sbox[c][d][0];
+                    const int e = sbox[c][d][1] - sbox[c][d][0] + 2;  // corners k .. k+1
+                    any = any && sbox[c][d][1] >= sbox[c][d][0];
+                    S.ext[c][d] = e;
+                    sz *= e > 0 ? e : 0;
+                }
+                S.off[c] = total;
+                total += sz;
+            }
+            if (any && total <= cap) {
+                for (int t = lt; t < total; t += kAdjBlock) tile[t] = 0.0;
+                S.tile = tile;
+            } else {
+                S.tile = nullptr;
+                total = 0;
+            }
+            __syncthreads();
+        }
+        if (live) {
             for (int j = n - 1; j >= 0; --j) {
                 double p0[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) p0[d] = tape[(j * D + d) * kAdjBlock + lt];
-                step_bw<D, M>(A, p0, pb, Vb);
+                step_bw<D, M>(A, p0, pb, S);
             }
         }
-    }
+        if constexpr (D == 2) {
+            __syncthreads();
+            // the tile's nonzero entries go to Vbar
+            for (int t = lt; t < total; t += kAdjBlock) {
+                const double v = tile[t];
+                if (v == 0.0) continue;
+                int c = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) x0b.p[d][i] = pb[d];
-    Tpb[i] = tb;
+                for (int cc = 1; cc < D; ++cc) c = t >= S.off[cc] ? cc : c;
+                int r = t - S.off[c];
+                int64_t gi = 0, stride = 1;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const int64_t node = S.lo[c][d] + r % S.ext[c][d];
+                    r /= S.ext[c][d];
+                    gi += node * stride;
+                    stride *= A.vg.ax[c][d].n;
+                }
+                unsafeAtomicAdd(S.g[c] + gi, v);
+            }
+            __syncthreads();  // the tile and the box are reused by the next block
+            S.tile = nullptr;
+        }
+    }
+    if (i < A.np) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) x0b.p[d][i] = pb[d];
+        Tpb[i] = tb;
+    }
 }
 
 template <int D>
@@ -703,9 +871,10 @@ int backward(jp_ctx* ctx, const AdjArgs<D>& A, const jp_velgrid* vg, const doubl
         xb.p[c] = x0bar[c];
     }
     if (A.np > 0) {
-        const size_t lds = (size_t)A.every * D * kAdjBlock * sizeof(double);
+        const int cap = tile_cap<D>(A.every);
+        const size_t lds = ((size_t)A.every * D * kAdjBlock + cap) * sizeof(double);
         k_adj_bwd<D, M><<<(unsigned)blocks_for(A.np, kAdjBlock), kAdjBlock, lds, ctx->stream>>>(
-            A, ckpt, alive, Tp, F, W, Fbar, vb, xb, Tpbar);
+            A, ckpt, alive, Tp, F, W, Fbar, vb, xb, Tpbar, cap);
         JP_LAUNCHED(ctx);
     }
     return finish(ctx);

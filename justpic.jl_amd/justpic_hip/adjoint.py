@@ -28,16 +28,22 @@ from .particles import CellArray, Particles
 
 
 def flatten(particles: Particles, field: CellArray | None = None):
-    """Active particles of a container as a flat list (slot-major order, the
-    order of the container's storage): D position arrays and the tracer taken
-    from ``field`` (zeros without one).  Host round trip; set-up only."""
+    """Active particles of a container as a flat list: D position arrays and
+    the tracer taken from ``field`` (zeros without one).  Order: groups of 64
+    consecutive cells, slot-major inside a group, so that a wavefront holds 64
+    different cells and a workgroup a compact patch (the pullback accumulates
+    a workgroup's velocity cotangents in LDS).  Host round trip; set-up only.
+    Any order gives the same model; only the speed of the pullback differs."""
     if getattr(particles, "form", "f64") != "f64":
         raise ValueError("flatten takes an FP64 container (convert with to_f64)")
     coords, index = particles.host_state()
-    m = index.astype(bool)
+    C, K = particles.C, particles.max_xcell
+    e = np.nonzero(index.astype(bool))[0]
+    c, k = e % C, e // C
+    e = e[np.lexsort((c % 64, k, c // 64))]
     ctx = particles.ctx
-    x = tuple(DeviceArray.from_host(ctx, np.ascontiguousarray(c[m])) for c in coords)
-    T = field.data.flat()[m] if field is not None else np.zeros(int(m.sum()))
+    x = tuple(DeviceArray.from_host(ctx, np.ascontiguousarray(a[e])) for a in coords)
+    T = field.data.flat()[e] if field is not None else np.zeros(e.size)
     return x, DeviceArray.from_host(ctx, np.ascontiguousarray(T, dtype=np.float64))
 
 
