@@ -22,8 +22,8 @@
 //    order of the oracle's restatement, so F and W are reproducible and equal
 //    to it bit for bit.
 //  - k_adj_bwd: blocks of steps in reverse order; a block is recomputed from
-//    its checkpoint into a per-thread LDS tape (the position at the start of
-//    each step), then reversed step by step.  x0bar and Tpbar are per-thread
+//    its checkpoint into a tape (the position at the start of each step, in
+//    global scratch), then reversed step by step.  x0bar and Tpbar are per-thread
 //    values.  In 2D the velocity cotangent of a workgroup and block of steps
 //    is accumulated in an LDS tile covering exactly the nodes the recompute
 //    touched (LDS atomic adds), then added to Vbar once; when that footprint
@@ -42,7 +42,7 @@ using namespace jp;
 
 namespace {
 
-constexpr int kAdjBlock = 128;
+constexpr int kAdjBlock = 512;  // backward workgroup: one 64-cell patch, 8 slot planes
 
 template <int D>
 struct AdjArgs {
@@ -587,27 +587,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_zero(double* __restrict__ p, int
 }
 
 // LDS tile for the velocity cotangents of one workgroup and block of steps
-// (2D only): whatever the 64 KB workgroup budget leaves after the tape.  A
-// workgroup of 128 particles from a compact patch touches ~2-4 k nodes over
-// ten steps; in 3D the footprint rarely fits and global atomics are used.
-constexpr size_t kLdsBudget = 65536 - 256;  // bytes, minus the static box words
+// (2D only), the 64 KB workgroup budget less the static box words.  A
+// workgroup from a compact patch touches ~2-6 k nodes over ten steps; in 3D
+// the footprint rarely fits and global atomics are used.
+constexpr size_t kLdsBudget = 65536 - 256;
 template <int D>
-inline int tile_cap(int every) {
-    if (D != 2) return 0;
-    const size_t tape = (size_t)every * D * kAdjBlock * sizeof(double);
-    return tape < kLdsBudget ? (int)((kLdsBudget - tape) / sizeof(double)) : 0;
+inline int tile_cap() {
+    return D == 2 ? (int)(kLdsBudget / sizeof(double)) : 0;
 }
 
 template <int D, int M>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_bwd(
     AdjArgs<D> A, const double* __restrict__ ckpt, const uint8_t* __restrict__ alive,
     const double* __restrict__ Tp, const double* __restrict__ F, const double* __restrict__ W,
-    const double* __restrict__ Fbar, Ptr3 Vb, Ptr3 x0b, double* __restrict__ Tpb, int cap) {
-    // dynamic LDS: the tape [every][D][kAdjBlock] of step-start positions,
-    // then the cotangent tile of cap doubles
-    extern __shared__ double smem[];
-    double* tape = smem;
-    double* tile = smem + A.every * D * kAdjBlock;
+    const double* __restrict__ Fbar, Ptr3 Vb, Ptr3 x0b, double* __restrict__ Tpb,
+    double* __restrict__ tape, int cap) {
+    // tape (global scratch, [every][D][np]): step-start positions of the block
+    // of steps being reversed; dynamic LDS: the cotangent tile of cap doubles
+    extern __shared__ double tile[];
     __shared__ int sbox[D][D][2];
     const int lt = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kAdjBlock + lt;
@@ -632,7 +629,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_bwd(
             for (int d = 0; d < D; ++d) x[d] = ckpt[((int64_t)b * D + d) * A.np + i];
             for (int j = 0; j < n; ++j) {
 #pragma unroll
-                for (int d = 0; d < D; ++d) tape[(j * D + d) * kAdjBlock + lt] = x[d];
+                for (int d = 0; d < D; ++d) tape[((int64_t)j * D + d) * A.np + i] = x[d];
                 double y[D];
                 step<D, M>(A, x, y, box);
 #pragma unroll
@@ -703,7 +700,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_bwd(
             for (int j = n - 1; j >= 0; --j) {
                 double p0[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) p0[d] = tape[(j * D + d) * kAdjBlock + lt];
+                for (int d = 0; d < D; ++d) p0[d] = tape[((int64_t)j * D + d) * A.np + i];
                 step_bw<D, M>(A, p0, pb, S);
             }
         }
@@ -871,10 +868,13 @@ int backward(jp_ctx* ctx, const AdjArgs<D>& A, const jp_velgrid* vg, const doubl
         xb.p[c] = x0bar[c];
     }
     if (A.np > 0) {
-        const int cap = tile_cap<D>(A.every);
-        const size_t lds = ((size_t)A.every * D * kAdjBlock + cap) * sizeof(double);
+        const int cap = tile_cap<D>();
+        const size_t lds = (size_t)cap * sizeof(double);
+        int rc = ensure_scratch(ctx, (size_t)A.every * D * A.np * sizeof(double));
+        if (rc) return rc;
+        double* tape = static_cast<double*>(ctx->scratch);
         k_adj_bwd<D, M><<<(unsigned)blocks_for(A.np, kAdjBlock), kAdjBlock, lds, ctx->stream>>>(
-            A, ckpt, alive, Tp, F, W, Fbar, vb, xb, Tpbar, cap);
+            A, ckpt, alive, Tp, F, W, Fbar, vb, xb, Tpbar, tape, cap);
         JP_LAUNCHED(ctx);
     }
     return finish(ctx);
