@@ -196,34 +196,6 @@ __device__ inline uint64_t active_slots(const uint8_t* __restrict__ index, int64
 // segment (q, wave) holds up to 64 cells at seg[(q*nwave + wave)*64 + i] and
 // its length at cnt[q*nwave + wave]; a compaction kernel joins the segments.
 // Every lane of the wavefront must call it (want = false: nothing to add).
-// The 8-slot chunks of a cell that hold a set bit of `bits`, in order, with
-// chunk i+1's loads issued before chunk i's stores: a load issued after a
-// store waits for it (the vector-memory counter counts stores too), so a
-// plain load-compute-store loop serialises a round trip per chunk behind the
-// previous chunk's stores.  load(k0, buf) issues loads only; proc(k0, buf)
-// computes and stores.
-template <class Buf, class Load, class Proc>
-__device__ inline void walk_chunks(uint64_t bits, int K, Load&& load, Proc&& proc) {
-    auto next = [&](int k) __attribute__((always_inline)) {
-        if (k >= K) return K;
-        const uint64_t r = bits >> k;
-        return r ? k + (__builtin_ctzll(r) & ~7) : K;
-    };
-    Buf cur, nxt;
-    int ka = next(0);
-    if (ka >= K) return;
-    load(ka, cur);
-#pragma unroll 1
-    for (;;) {
-        const int kb = next(ka + 8);
-        if (kb < K) load(kb, nxt);
-        proc(ka, cur);
-        if (kb >= K) return;
-        cur = nxt;  // register copy; waits only for nxt's loads
-        ka = kb;
-    }
-}
-
 // Nearest-neighbour search of the injection on squared distances.
 // wave_argmin: lexicographic minimum of (s, order) over the wavefront.
 __device__ inline void wave_argmin(double& best, int& border) {

@@ -84,10 +84,9 @@ __device__ inline int phase_of(const int64_t* I) {
 // Fields of the movers of one chunk of 8 slots, copied to the staging area in
 // one batch of independent loads (stg: chunk-local mover bits).
 __device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint32_t stg,
-                                  int a_first, int nargs, const ArgPtrs& args,
-                                  const MoveScratch& ms) {
+                                  int nargs, const ArgPtrs& args, const MoveScratch& ms) {
     if (!stg) return;
-    for (int a = a_first; a < nargs; ++a) {
+    for (int a = 0; a < nargs; ++a) {
         const double* __restrict__ src = args.p[a];
         double* __restrict__ dst = ms.stga + a * S;
         double v[8];
@@ -135,27 +134,17 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
     uint64_t mov = 0;
     double* __restrict__ sc = static_cast<double*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
-    const double* __restrict__ a0 = nargs > 0 ? args.p[0] : nullptr;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
-    // loads and stores across neighbouring cells); the coordinates and the
-    // first field of a chunk's particles are loaded together, a chunk ahead
+    // loads and stores across neighbouring cells)
     constexpr int B = 8;
-    struct Buf {
+    for (int k0 = 0; k0 < g.K; k0 += B) {
         double pb[B][D];
-        double fa[B];
-    };
-    auto load = [&](int k0, Buf& b) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (occ >> (k0 + j)) & 1;
-            const int64_t e = c + C * (k0 + j);
 #pragma unroll
-            for (int d = 0; d < D; ++d) b.pb[j][d] = on ? coords.p[d][e] : 0.0;
-            b.fa[j] = (on && a0) ? a0[e] : 0.0;
+            for (int d = 0; d < D; ++d) pb[j][d] = on ? coords.p[d][c + C * (k0 + j)] : 0.0;
         }
-    };
-    auto proc = [&](int k0, const Buf& b) __attribute__((always_inline)) {
-        const auto& pb = b.pb;
         uint64_t word = 0;
         uint32_t mv8 = 0, stg = 0;
 #pragma unroll
@@ -204,7 +193,6 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     if (n != kCenter<D> && !inJ) cd |= kGap;
 #pragma unroll
                     for (int d = 0; d < D; ++d) sc[d * S + c + C * k] = p[d];
-                    if (a0) ms.stga[c + C * k] = b.fa[j];
                     stg |= 1u << j;
                 }
             }
@@ -212,9 +200,8 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args(c, C, S, k0, stg, 1, nargs, args, ms);
-    };
-    walk_chunks<Buf>(occ, g.K, load, proc);
+        stage_args(c, C, S, k0, stg, nargs, args, ms);
+    }
     ms.occ0[c] = occ;
     ms.occ[c] = occ;
     ms.mov[c] = mov;
@@ -620,24 +607,15 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
     uint64_t mov = 0;
     float* __restrict__ sc = static_cast<float*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
-    const double* __restrict__ a0 = nargs > 0 ? args.p[0] : nullptr;
     constexpr int B = 8;
-    struct Buf {
+    for (int k0 = 0; k0 < g.K; k0 += B) {
         float ub[B][D];
-        double fa[B];
-    };
-    auto load = [&](int k0, Buf& b) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (occ >> (k0 + j)) & 1;
-            const int64_t e = c + C * (k0 + j);
 #pragma unroll
-            for (int d = 0; d < D; ++d) b.ub[j][d] = on ? off.p[d][e] : 0.0f;
-            b.fa[j] = (on && a0) ? a0[e] : 0.0;
+            for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + C * (k0 + j)] : 0.0f;
         }
-    };
-    auto proc = [&](int k0, const Buf& b) __attribute__((always_inline)) {
-        const auto& ub = b.ub;
         uint64_t word = 0;
         uint32_t mv8 = 0, stg = 0;
 #pragma unroll
@@ -672,16 +650,14 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
 #pragma unroll
                 for (int d = 0; d < D; ++d)
                     sc[d * S + c + C * k] = (float)((double)ub[j][d] - (double)s[d]);
-                if (a0) ms.stga[c + C * k] = b.fa[j];
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args(c, C, S, k0, stg, 1, nargs, args, ms);
-    };
-    walk_chunks<Buf>(occ, g.K, load, proc);
+        stage_args(c, C, S, k0, stg, nargs, args, ms);
+    }
     ms.occ0[c] = occ;
     ms.occ[c] = occ;
     ms.mov[c] = mov;
