@@ -105,7 +105,7 @@ struct Tile {
     }
 };
 template <int D>
-using DefaultTile = typename std::conditional<D == 2, Tile<2, 64, 4>, Tile<3, 16, 4, 4>>::type;
+using DefaultTile = typename std::conditional<D == 2, Tile<2, 64, 4>, Tile<3, 32, 4, 2>>::type;
 
 // FP64 input velocity (component grids n[c][d]) and the vertex spacings
 template <int D>
