@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_from_split(Grid<D> g, FPtr3 u, const
 }
 
 // ------------------------------------------------------------ advection
-// One workgroup per tile of cells (2D 64 x 4, 3D 16 x 4 x 4), one thread per
+// One workgroup per tile of cells (2D 64 x 4, 3D 32 x 4 x 2), one thread per
 // cell.  The tile's window of every velocity component, nodes [t0 - 1,
 // t0 + T + 3) per dimension, is staged in LDS already scaled to cells per
 // step, float(V_c * dt / h_c).  That window covers every stage point within one
