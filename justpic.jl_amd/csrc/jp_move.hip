@@ -227,6 +227,12 @@ template <int D, int BS>
 __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3 off, int phase,
                                                  const int* __restrict__ flag, jp_stats* st) {
     __shared__ uint64_t s_occ[kNB<D>][BS];
+    // 2D: the cell's code words and its neighbour cells, read per mover from
+    // LDS instead of selected from registers (the phase is one wave's serial
+    // chain per SIMD, so every instruction of the mover loop counts)
+    constexpr bool kTab = D == 2;
+    __shared__ uint64_t s_cw[kTab ? 8 : 1][BS];
+    __shared__ int64_t s_dc[kTab ? kNB<D> : 1][BS];
     const int stop = *flag;  // checked below, after this cell's loads are issued
     int64_t ncol[D];
     int64_t total = 1;
@@ -284,6 +290,12 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     if (stop || !pend) return;  // serial fallback, or nothing leaves or re-slots here
 #pragma unroll
     for (int n = 0; n < kNB<D>; ++n) s_occ[n][tid] = was[n];
+    if constexpr (kTab) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s_cw[j][tid] = cw[j];
+#pragma unroll
+        for (int n = 0; n < kNB<D>; ++n) s_dc[n][tid] = nbc[n];
+    }
     uint64_t selfm = 0;  // slots that received a re-slotted particle of this cell
     uint64_t selfins = 0;
     int sp = 0;
@@ -292,11 +304,16 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
         const int ip = __builtin_ctzll(pend);
         pend &= pend - 1;
         const uint64_t bit = 1ull << ip;
-        uint64_t wsel = cw[0];
+        uint8_t code;
+        if constexpr (kTab) {
+            code = reinterpret_cast<const uint8_t*>(&s_cw[ip >> 3][tid])[ip & 7];
+        } else {
+            uint64_t wsel = cw[0];
 #pragma unroll
-        for (int j = 1; j < 8; ++j) wsel = ((ip >> 3) == j) ? cw[j] : wsel;
-        const uint8_t cd =
-            (selfm & bit) ? (uint8_t)kCenter<D> : (uint8_t)(wsel >> (8 * (ip & 7)));
+            for (int j = 1; j < 8; ++j) wsel = ((ip >> 3) == j) ? cw[j] : wsel;
+            code = (uint8_t)(wsel >> (8 * (ip & 7)));
+        }
+        const uint8_t cd = (selfm & bit) ? (uint8_t)kCenter<D> : code;
         const uint32_t origin = (arrS & bit) ? ms.org[S + C * ip] : (uint32_t)(S + C * ip);
         s_occ[kCenter<D>][tid] &= ~bit;  // empty_particle! on the source slot
         arrS &= ~bit;
@@ -317,7 +334,9 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
         s_occ[n][tid] = od | (1ull << f);
         ++n_moved;
         int64_t Dc = S;
-        {
+        if constexpr (kTab) {
+            Dc = s_dc[n][tid];
+        } else {
             int r = n;
             int64_t stride = 1;
 #pragma unroll
