@@ -442,6 +442,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--method", choices=("rk2", "rk4"), default=None,
+                    help="integrator (default: RK2 for step, RK4 for step3d)")
     ap.add_argument("--cpu-steps", type=int, default=6,
                     help="oracle steps timed for the CPU baseline (0 = skip)")
     args = ap.parse_args()
@@ -471,7 +473,10 @@ def main():
         run_adjoint(jp, args)
         return
     D = 3 if args.case == "step3d" else 2
-    spec = WORKLOADS[D]
+    spec = dict(WORKLOADS[D])
+    default_method = spec["method"]
+    if args.method:
+        spec["method"] = args.method
     args.n = args.n or (128 if D == 3 else 512)
     log("building workload", D, "D", args.n)
     if D == 2:
@@ -493,6 +498,8 @@ def main():
     # roofline: the advection kernel (one launch per call)
     adv = main_r["calls"]["advection"]
     tag = f"{args.n}" if D == 2 else f"{args.n}_3d"
+    if spec["method"] != default_method:
+        tag += f"_{spec['method']}"
     traffic = read_traffic(os.path.join(ROOT, "profiles",
                                         f"traffic_advection_{args.form}_{tag}.txt"))
     meth = spec["method"].upper()
@@ -517,10 +524,10 @@ def main():
     names = {"split": "FP32-split positions", "fp64": "FP64 drop-in"}
     if D == 2:
         workload = (f"2D temperature advection {size} cells, nxcell 24, max_xcell 48, "
-                    f"min_xcell 12, RK2 + move + inject + p2g, {names[args.form]}")
+                    f"min_xcell 12, {meth} + move + inject + p2g, {names[args.form]}")
     else:
-        workload = (f"3D RK4 advection {size} cells, cellular flow x100, nxcell 27, "
-                    f"max_xcell 54, min_xcell 14, RK4 + move + p2g of one field, "
+        workload = (f"3D {meth} advection {size} cells, cellular flow x100, nxcell 27, "
+                    f"max_xcell 54, min_xcell 14, {meth} + move + p2g of one field, "
                     f"{names[args.form]}")
     out = {
         "metric": METRIC,

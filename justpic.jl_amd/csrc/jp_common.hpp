@@ -192,6 +192,25 @@ __device__ inline uint64_t active_slots(const uint8_t* __restrict__ index, int64
     return bits;
 }
 
+// Active slots kb..ke-1 of cell c (bits at their slot positions), in rounds
+// of 16 clamped loads.
+__device__ inline uint64_t active_slots_in(const uint8_t* __restrict__ index, int64_t c,
+                                           int64_t C, int kb, int ke) {
+    uint64_t bits = 0;
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        uint8_t m[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int k = k0 + v < ke ? k0 + v : ke - 1;
+            m[v] = index[c + C * k];
+        }
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            if (k0 + v < ke) bits |= (uint64_t)(m[v] != 0) << (k0 + v);
+    }
+    return bits;
+}
+
 // Appends c to the list segment of its wavefront and colour without atomics:
 // segment (q, wave) holds up to 64 cells at seg[(q*nwave + wave)*64 + i] and
 // its length at cnt[q*nwave + wave]; a compaction kernel joins the segments.

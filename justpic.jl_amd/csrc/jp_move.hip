@@ -98,6 +98,34 @@ __device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint3
     }
 }
 
+// The per-cell words of slot group kb..ke-1 (kb, and ke unless it is K, are
+// multiples of 8): whole words for a single group, else the group's bytes
+// (the last group also writes the zero bytes above K).
+__device__ inline void store_cell_words(const MoveScratch& ms, int64_t c, int kb, int ke, int K,
+                                        uint64_t occ, uint64_t mov) {
+    if (kb == 0 && ke == K) {
+        ms.occ0[c] = occ;
+        ms.occ[c] = occ;
+        ms.mov[c] = mov;
+        ms.arr[c] = 0;
+        ms.tch[c] = 0;
+        return;
+    }
+    uint8_t* __restrict__ o0 = reinterpret_cast<uint8_t*>(ms.occ0 + c);
+    uint8_t* __restrict__ o1 = reinterpret_cast<uint8_t*>(ms.occ + c);
+    uint8_t* __restrict__ mv = reinterpret_cast<uint8_t*>(ms.mov + c);
+    const int b1 = ke == K ? 8 : ke >> 3;
+    for (int b = kb >> 3; b < b1; ++b) {
+        o0[b] = (uint8_t)(occ >> (8 * b));
+        o1[b] = (uint8_t)(occ >> (8 * b));
+        mv[b] = (uint8_t)(mov >> (8 * b));
+    }
+    if (kb == 0) {
+        ms.arr[c] = 0;
+        ms.tch[c] = 0;
+    }
+}
+
 // Destination codes are packed 8 per 64-bit word, word j of cell c at word
 // index j*C + c (coalesced stores here), so the colour phases load a cell's
 // codes in one round of word loads.
@@ -105,9 +133,11 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                                                         const uint8_t* __restrict__ index,
                                                         int nargs, ArgPtrs args, MoveScratch ms,
-                                                        int* flag) {
+                                                        int* flag, int span) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= g.C) return;
+    const int kb = blockIdx.y * span;  // slot group kb..ke-1
+    const int ke = kb + span < g.K ? kb + span : g.K;
     const int64_t C = g.C, S = C * g.K;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
@@ -130,14 +160,15 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
         dlo[d] = A.x[0];
         dhi[d] = A.x[n - 1];
     }
-    const uint64_t occ = active_slots(index, c, C, g.K);
+    const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                : active_slots_in(index, c, C, kb, ke);
     uint64_t mov = 0;
     double* __restrict__ sc = static_cast<double*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
     // loads and stores across neighbouring cells)
     constexpr int B = 8;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
+    for (int k0 = kb; k0 < ke; k0 += B) {
         double pb[B][D];
 #pragma unroll
         for (int j = 0; j < B; ++j) {
@@ -202,11 +233,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
         mov |= (uint64_t)mv8 << k0;
         stage_args(c, C, S, k0, stg, nargs, args, ms);
     }
-    ms.occ0[c] = occ;
-    ms.occ[c] = occ;
-    ms.mov[c] = mov;
-    ms.arr[c] = 0;
-    ms.tch[c] = 0;
+    store_cell_words(ms, c, kb, ke, g.K, occ, mov);
 }
 
 // ------------------------------------------------------------ 2. colour phases
@@ -401,20 +428,23 @@ template <int D, typename T>
 __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> coords,
                                                        uint8_t* __restrict__ index, int nargs,
                                                        ArgPtrs args, MoveScratch ms,
-                                                       const int* __restrict__ flag) {
-    // one thread per cell, chunks of 8 slot planes in lockstep across the
-    // wavefront (coalesced origin loads and final stores); only the reads of
-    // the staged data are scattered.  Arrivals copy their staged data, vacated
-    // slots are emptied (empty_particle!, src/Particles/move_safe.jl:243-251).
+                                                       const int* __restrict__ flag, int span) {
+    // one thread per cell and slot group (blockIdx.y, `span` slots), chunks of
+    // 8 slot planes in lockstep across the wavefront (coalesced origin loads
+    // and final stores); only the reads of the staged data are scattered.
+    // Arrivals copy their staged data, vacated slots are emptied
+    // (empty_particle!, src/Particles/move_safe.jl:243-251).
     if (*flag) return;
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= g.C) return;
+    const int kb = blockIdx.y * span;
+    const int ke = kb + span < g.K ? kb + span : g.K;
     const uint64_t arr = ms.arr[c], occ = ms.occ[c], occ0 = ms.occ0[c], tch = ms.tch[c];
     const uint64_t work = arr | (~arr & ~occ & (occ0 | tch));
-    if (!work) return;
+    if (!(work >> kb)) return;
     const int64_t C = g.C, S = C * g.K;
     const T* __restrict__ sc = static_cast<const T*>(ms.stgc);
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
+    for (int k0 = kb; k0 < ke; k0 += 8) {
         const uint32_t wb = (uint32_t)(work >> k0) & 0xFFu;
         if (!wb) continue;
         const uint32_t ab = (uint32_t)(arr >> k0) & 0xFFu;
@@ -616,18 +646,21 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D, float> off,
                                                               const uint8_t* __restrict__ index,
                                                               int nargs, ArgPtrs args,
-                                                              MoveScratch ms, int* flag) {
+                                                              MoveScratch ms, int* flag, int span) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= g.C) return;
+    const int kb = blockIdx.y * span;  // slot group kb..ke-1
+    const int ke = kb + span < g.K ? kb + span : g.K;
     const int64_t C = g.C, S = C * g.K;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
-    const uint64_t occ = active_slots(index, c, C, g.K);
+    const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                : active_slots_in(index, c, C, kb, ke);
     uint64_t mov = 0;
     float* __restrict__ sc = static_cast<float*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     constexpr int B = 8;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
+    for (int k0 = kb; k0 < ke; k0 += B) {
         float ub[B][D];
 #pragma unroll
         for (int j = 0; j < B; ++j) {
@@ -677,11 +710,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
         mov |= (uint64_t)mv8 << k0;
         stage_args(c, C, S, k0, stg, nargs, args, ms);
     }
-    ms.occ0[c] = occ;
-    ms.occ[c] = occ;
-    ms.mov[c] = mov;
-    ms.arr[c] = 0;
-    ms.tch[c] = 0;
+    store_cell_words(ms, c, kb, ke, g.K, occ, mov);
 }
 
 ArgPtrs argptrs(double* const* a, int n) {
@@ -737,14 +766,18 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     ArgPtrs ap = argptrs(args, nargs);
     TPtr3<D, T> cp = tptr3<D, T>(coords);
+    // per-cell kernels: slot groups on blockIdx.y when the cells alone do not
+    // fill the chip
+    const int span = slot_span(blocks_for(C), g.K);
+    const dim3 cgrid((unsigned)blocks_for(C), (unsigned)((g.K + span - 1) / span));
     if constexpr (std::is_same<T, double>::value) {
         CPtr3 cc{{nullptr, nullptr, nullptr}};
         for (int d = 0; d < D; ++d) cc.p[d] = coords[d];
-        k_mv_classify<D><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms,
-                                                                   flag);
+        k_mv_classify<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms, flag,
+                                                           span);
     } else {
-        k_mv_classify_split<D><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap,
-                                                                         ms, flag);
+        k_mv_classify_split<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
+                                                                 flag, span);
     }
     JP_LAUNCHED(ctx);
     int64_t total = 1;
@@ -764,8 +797,8 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     }
     k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
     JP_LAUNCHED(ctx);
-    k_mv_scatter<D, T><<<blocks_for(C), kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
-                                                                 flag);
+    k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, flag,
+                                                         span);
     JP_LAUNCHED(ctx);
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);

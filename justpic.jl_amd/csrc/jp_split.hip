@@ -222,12 +222,13 @@ __device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVe
     }
 }
 
-// advection of the particles of tile `tile` of one grid (all threads of the
-// block call it; the LDS window sV is the block's)
+// advection of the particles in slots kb..ke-1 of tile `tile` of one grid (all
+// threads of the block call it; the LDS window sV is the block's)
 template <int D, int M, class TS>
 __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, double dt,
                                       const FPtr3& off, const uint8_t* __restrict__ index,
-                                      const RKf& P, int tile, float (*sV)[TS::size()]) {
+                                      const RKf& P, int tile, int kb, int ke,
+                                      float (*sV)[TS::size()]) {
     int t0[D];
     {
         int b = tile;
@@ -287,9 +288,10 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
 #pragma unroll
     for (int d = D - 2; d >= 0; --d) c = c * g.ni[d] + I[d];
     const int64_t C = g.C;
-    const uint64_t act = active_slots(index, c, C, g.K);
+    const uint64_t act = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                : active_slots_in(index, c, C, kb, ke);
     constexpr int B = 8;
-    for (int k0 = 0; k0 < g.K; k0 += B) {
+    for (int k0 = kb; k0 < ke; k0 += B) {
         const uint32_t ab = (uint32_t)(act >> k0) & 0xFFu;
         if (!ab) continue;
         float ub[B][D];
@@ -317,13 +319,17 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
     }
 }
 
+// blockIdx.y: slot group of `span` slots (a multiple of 8), so that grids with
+// few tiles still fill the chip; each group stages its own velocity window
 template <int D, int M>
 __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, double dt,
                                                       FPtr3 off, const uint8_t* __restrict__ index,
-                                                      RKf P) {
+                                                      RKf P, int span) {
     using TS = DefaultTile<D>;
     __shared__ float sV[D][TS::size()];
-    adv_split_tile<D, M, TS>(g, V, dt, off, index, P, blockIdx.x, sV);
+    const int kb = blockIdx.y * span;
+    const int ke = kb + span < g.K ? kb + span : g.K;
+    adv_split_tile<D, M, TS>(g, V, dt, off, index, P, blockIdx.x, kb, ke, sV);
 }
 
 // ------------------------------------------------------------ ensemble
@@ -349,6 +355,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split_ens(const EnsGrid* __restr
         const EnsGrid& E = grids[i];
         // zero_dt: the autotuner's timing runs (same work, positions unchanged)
         adv_split_tile<2, M, TS>(E.g, E.V, zero_dt ? 0.0 : E.dt, E.off, E.index, P, t - tile0[i],
+                                 0, E.g.K,
                                  sV);
         __syncthreads();  // the window is refilled for the next tile
     }
@@ -967,12 +974,14 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     for (int d = 0; d < D; ++d)
         tiles *= (G.ni[d] + DefaultTile<D>::n(d) - 1) / DefaultTile<D>::n(d);
     FPtr3 fp = fptr3<D>(off);
+    const int span = slot_span(tiles, G.K);
+    const dim3 grid((unsigned)tiles, (unsigned)((G.K + span - 1) / span));
     if (method == JP_EULER)
-        k_adv_split<D, JP_EULER><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
+        k_adv_split<D, JP_EULER><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P, span);
     else if (method == JP_RK2)
-        k_adv_split<D, JP_RK2><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
+        k_adv_split<D, JP_RK2><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P, span);
     else
-        k_adv_split<D, JP_RK4><<<(unsigned)tiles, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P);
+        k_adv_split<D, JP_RK4><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P, span);
     JP_LAUNCHED(ctx);
     return finish(ctx);
 }
