@@ -32,6 +32,8 @@ namespace {
 
 constexpr uint8_t kDel = 0xFE;
 constexpr uint8_t kGap = 0x80;  // arrival fails the strict in-cell test of its destination
+// s_waitcnt immediate of gfx9 for vmcnt(0) with expcnt and lgkmcnt left free
+constexpr int kVmcnt0 = 0x0F70;
 
 __device__ inline double dnan() { return __builtin_nan(""); }
 
@@ -341,7 +343,14 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
             code = (uint8_t)(wsel >> (8 * (ip & 7)));
         }
         const uint8_t cd = (selfm & bit) ? (uint8_t)kCenter<D> : code;
-        const uint32_t origin = (arrS & bit) ? ms.org[S + C * ip] : (uint32_t)(S + C * ip);
+        uint32_t origin = (uint32_t)(S + C * ip);
+        if (arrS & bit) {  // rare: an arrival of this call moves on
+            origin = ms.org[S + C * ip];
+            // wait here, on the rare path: otherwise the loop head waits for
+            // the load on every iteration, and with it (one counter for loads
+            // and stores) for the previous iteration's store
+            __builtin_amdgcn_s_waitcnt(kVmcnt0);
+        }
         s_occ[kCenter<D>][tid] &= ~bit;  // empty_particle! on the source slot
         arrS &= ~bit;
         selfm &= ~bit;
