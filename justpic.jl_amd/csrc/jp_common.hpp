@@ -192,6 +192,14 @@ __device__ inline uint64_t active_slots(const uint8_t* __restrict__ index, int64
     return bits;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup w to XCD w % 8),
+// each XCD with its own L2.  Renumbered by this bijection, XCD x runs one
+// contiguous range of logical workgroups, so neighbouring cells share an L2.
+__device__ inline unsigned xcd_block(unsigned w, unsigned n) {
+    const unsigned q = n / 8, r = n % 8, x = w % 8;
+    return x * q + (x < r ? x : r) + w / 8;
+}
+
 // Active slots kb..ke-1 of cell c (bits at their slot positions), in rounds
 // of 16 clamped loads.
 __device__ inline uint64_t active_slots_in(const uint8_t* __restrict__ index, int64_t c,

@@ -83,10 +83,11 @@ inline int64_t blocks_for(int64_t n, int b = kBlock) { return (n + b - 1) / b; }
 
 // Slots per slot group (a multiple of 8) for a per-cell kernel of `blocks`
 // workgroups over K slots: groups are added (blockIdx.y) until the launch has
-// about 8 workgroups per CU of the 256, at most one group per 8 slots.
-inline int slot_span(int64_t blocks, int K) {
+// about `target` workgroups (default 8 per CU of the 256), at most one group
+// per 8 slots.
+inline int slot_span(int64_t blocks, int K, int64_t target = 2048) {
     const int64_t chunks = (K + 7) / 8;
-    int64_t g = (2048 + blocks - 1) / (blocks > 0 ? blocks : 1);
+    int64_t g = (target + blocks - 1) / (blocks > 0 ? blocks : 1);
     g = g < 1 ? 1 : (g > chunks ? chunks : g);
     const int64_t per = (chunks + g - 1) / g;
     return (int)(per * 8);

@@ -444,9 +444,19 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
     // Arrivals copy their staged data, vacated slots are emptied
     // (empty_particle!, src/Particles/move_safe.jl:243-251).
     if (*flag) return;
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // 2D: XCD-contiguous cell ranges (a destination's sources are its
+    // neighbour cells, whose staged lines other threads of the same L2 also
+    // read; a row of cells above or below is only a few workgroups away, which
+    // round-robin dealing puts on other XCDs); slot groups of one cell block
+    // stay together.  3D keeps the dealt order: there the plane above or below
+    // is a multiple of 8 workgroups away, already on the same XCD, and the
+    // renumbering measured slower.
+    const unsigned lb = D == 2 ? xcd_block(blockIdx.x + gridDim.x * blockIdx.y,
+                                           gridDim.x * gridDim.y)
+                               : blockIdx.x * gridDim.y + blockIdx.y;
+    const int64_t c = (lb / gridDim.y) * (int64_t)blockDim.x + threadIdx.x;
     if (c >= g.C) return;
-    const int kb = blockIdx.y * span;
+    const int kb = (int)(lb % gridDim.y) * span;
     const int ke = kb + span < g.K ? kb + span : g.K;
     const uint64_t arr = ms.arr[c], occ = ms.occ[c], occ0 = ms.occ0[c], tch = ms.tch[c];
     const uint64_t work = arr | (~arr & ~occ & (occ0 | tch));
@@ -776,8 +786,9 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     ArgPtrs ap = argptrs(args, nargs);
     TPtr3<D, T> cp = tptr3<D, T>(coords);
     // per-cell kernels: slot groups on blockIdx.y when the cells alone do not
-    // fill the chip
-    const int span = slot_span(blocks_for(C), g.K);
+    // fill the chip (up to 24 workgroups per CU: with one 8-slot chunk per
+    // thread, no chunk's loads wait for an earlier chunk's stores)
+    const int span = slot_span(blocks_for(C), g.K, 6144);
     const dim3 cgrid((unsigned)blocks_for(C), (unsigned)((g.K + span - 1) / span));
     if constexpr (std::is_same<T, double>::value) {
         CPtr3 cc{{nullptr, nullptr, nullptr}};
