@@ -412,8 +412,13 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
                 ms.arr[nbc[n]] = arrN[n] | ins;
                 ms.tch[nbc[n]] = tchN[n] | ins;
             } else {
-                ms.arr[nbc[n]] |= ins;
-                ms.tch[nbc[n]] |= ins;
+                // no other thread of the phase touches these words: an atomic
+                // OR without return updates them with no load to wait for (a
+                // load here would also wait for every earlier store)
+                atomicOr(reinterpret_cast<unsigned long long*>(ms.arr + nbc[n]),
+                         (unsigned long long)ins);
+                atomicOr(reinterpret_cast<unsigned long long*>(ms.tch + nbc[n]),
+                         (unsigned long long)ins);
             }
         }
     }
