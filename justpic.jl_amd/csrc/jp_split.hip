@@ -299,7 +299,11 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
         for (int j = 0; j < B; ++j)
 #pragma unroll
             for (int d = 0; d < D; ++d) ub[j][d] = ((ab >> j) & 1) ? off.p[d][c + C * (k0 + j)] : 0.0f;
-        // one particle at a time (the chunk's loads are already in flight)
+        // one particle at a time (the chunk's loads are already in flight);
+        // the results go back into ub and are stored plane by plane after the
+        // chunk, so that each store instruction covers one slot plane of the
+        // wavefront's cells (a store per particle would scatter a wavefront's
+        // lanes over the chunk's planes)
         uint32_t todo = ab;
         while (todo) {
             const int j = __builtin_ctz(todo);
@@ -314,8 +318,15 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
             }
             advect_split_one<D, M, TS>(sV, V, g, sc, t0, I, P, u0, out);
 #pragma unroll
-            for (int d = 0; d < D; ++d) off.p[d][c + C * (k0 + j)] = out[d];
+            for (int q = 0; q < B; ++q)
+#pragma unroll
+                for (int d = 0; d < D; ++d) ub[q][d] = j == q ? out[d] : ub[q][d];
         }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if ((ab >> j) & 1)
+#pragma unroll
+                for (int d = 0; d < D; ++d) off.p[d][c + C * (k0 + j)] = ub[j][d];
     }
 }
 
