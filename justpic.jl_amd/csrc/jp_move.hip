@@ -44,10 +44,21 @@ struct MoveScratch {
     uint64_t* arr;   // slots holding a particle that arrived during this call
     uint64_t* tch;   // slots that received a particle at some point
     uint8_t* code;   // per slot, 8 per word (word j*C + c): neighbour code / kDel, movers only
-    uint32_t* org;   // per slot (plane-major): original slot index of an arrival
+    uint32_t* org;   // per slot: original slot index of an arrival (layout: org_at)
     void* stgc;      // D planes of C*K (coordinate type): staged mover coordinates
     double* stga;    // nargs planes of C*K: staged mover fields, by original slot
 };
+
+// Position of (cell, slot) in the origin array.  2D: plane-major like the
+// particle data (c + C k).  3D: cell-major (c K + k): each mover of a colour
+// phase stores one origin into a neighbour cell, and cell-major keeps a
+// wavefront's stores inside the few lines of its cells' neighbourhoods
+// (plane-major scatters them over an array larger than the last-level cache:
+// 452 MB at 128^3 with K = 54).
+template <int D>
+__device__ inline uint32_t org_at(uint32_t c, uint32_t k, uint32_t C, uint32_t K) {
+    return D == 2 ? c + C * k : c * K + k;
+}
 
 template <int D>
 __device__ inline bool in_cell_at(const Grid<D>& g, const int64_t* I, const double* p) {
@@ -255,13 +266,19 @@ __device__ inline int64_t nb_cell(const Grid<D>& g, const int64_t* I, int n, boo
 template <int D, int BS>
 __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3 off, int phase,
                                                  const int* __restrict__ flag, jp_stats* st) {
-    __shared__ uint64_t s_occ[kNB<D>][BS];
-    // 2D: the cell's code words and its neighbour cells, read per mover from
-    // LDS instead of selected from registers (the phase is one wave's serial
-    // chain per SIMD, so every instruction of the mover loop counts)
-    constexpr bool kTab = D == 2;
-    __shared__ uint64_t s_cw[kTab ? 8 : 1][BS];
-    __shared__ int64_t s_dc[kTab ? kNB<D> : 1][BS];
+    // Per thread in LDS: for each cell of the 3^D neighbourhood its occupancy
+    // word and cell number (one 16-byte entry), and the cell's code words.
+    // The mover loop is one serial chain per thread with under two wavefronts
+    // per SIMD, so it is kept short: per mover one LDS byte read (the code),
+    // then one read of the destination's entry, 32-bit slot arithmetic (a move
+    // has fewer than 2^32 slots), and this cell's own word in a register.
+    struct alignas(16) NbEntry {
+        uint64_t occ;
+        uint32_t cell;  // 0xFFFFFFFF outside the grid (never a destination)
+        uint32_t pad;
+    };
+    __shared__ NbEntry s_nb[kNB<D>][BS];
+    __shared__ uint64_t s_cw[8][BS];
     const int stop = *flag;  // checked below, after this cell's loads are issued
     int64_t ncol[D];
     int64_t total = 1;
@@ -285,6 +302,7 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     const int tid = threadIdx.x;
     const int K = g.K;
     const int64_t C = g.C;
+    const uint32_t S32 = (uint32_t)S, C32 = (uint32_t)C;
     const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
     // everything this cell needs, in one round of independent loads: its
     // mover/arrival words and code words, and occupancy words of its 3^D
@@ -292,7 +310,7 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     // Cells of one colour are 3 apart, so no other thread of this phase reads
     // or writes these neighbourhood words and the values loaded now are still
     // current at the end (loading them late waits for this thread's earlier
-    // stores).  In 3D the 27 extra pairs would cost the occupancy they save.
+    // stores).  In 3D the 26 extra pairs would cost the occupancy they save.
     constexpr bool kPre = D == 2;
     uint64_t pend = ms.mov[S];
     uint64_t arrS = ms.arr[S];
@@ -318,40 +336,29 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     }
     if (stop || !pend) return;  // serial fallback, or nothing leaves or re-slots here
 #pragma unroll
-    for (int n = 0; n < kNB<D>; ++n) s_occ[n][tid] = was[n];
-    if constexpr (kTab) {
+    for (int n = 0; n < kNB<D>; ++n) s_nb[n][tid] = NbEntry{was[n], (uint32_t)nbc[n], 0u};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) s_cw[j][tid] = cw[j];
-#pragma unroll
-        for (int n = 0; n < kNB<D>; ++n) s_dc[n][tid] = nbc[n];
-    }
+    for (int j = 0; j < 8; ++j) s_cw[j][tid] = cw[j];
+    uint64_t occC = was[kCenter<D>];  // this cell's occupancy
     uint64_t selfm = 0;  // slots that received a re-slotted particle of this cell
     uint64_t selfins = 0;
     int sp = 0;
-    int64_t n_moved = 0, n_del = 0, n_drop = 0;
+    int n_moved = 0, n_del = 0, n_drop = 0;
     while (pend) {
         const int ip = __builtin_ctzll(pend);
         pend &= pend - 1;
         const uint64_t bit = 1ull << ip;
-        uint8_t code;
-        if constexpr (kTab) {
-            code = reinterpret_cast<const uint8_t*>(&s_cw[ip >> 3][tid])[ip & 7];
-        } else {
-            uint64_t wsel = cw[0];
-#pragma unroll
-            for (int j = 1; j < 8; ++j) wsel = ((ip >> 3) == j) ? cw[j] : wsel;
-            code = (uint8_t)(wsel >> (8 * (ip & 7)));
-        }
-        const uint8_t cd = (selfm & bit) ? (uint8_t)kCenter<D> : code;
-        uint32_t origin = (uint32_t)(S + C * ip);
+        const uint8_t code = reinterpret_cast<const uint8_t*>(&s_cw[ip >> 3][tid])[ip & 7];
+        uint32_t origin = S32 + C32 * (uint32_t)ip;
         if (arrS & bit) {  // rare: an arrival of this call moves on
-            origin = ms.org[S + C * ip];
+            origin = ms.org[org_at<D>(S32, (uint32_t)ip, C32, (uint32_t)K)];
             // wait here, on the rare path: otherwise the loop head waits for
             // the load on every iteration, and with it (one counter for loads
             // and stores) for the previous iteration's store
             __builtin_amdgcn_s_waitcnt(kVmcnt0);
         }
-        s_occ[kCenter<D>][tid] &= ~bit;  // empty_particle! on the source slot
+        const uint8_t cd = (selfm & bit) ? (uint8_t)kCenter<D> : code;
+        occC &= ~bit;  // empty_particle! on the source slot
         arrS &= ~bit;
         selfm &= ~bit;
         if (cd == kDel) {
@@ -359,70 +366,67 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
             continue;
         }
         const int n = cd & 0x7F;
-        const uint64_t od = s_occ[n][tid];
+        const NbEntry nb = s_nb[n][tid];
+        const uint32_t dcn = nb.cell;
+        const uint64_t od = n == kCenter<D> ? occC : nb.occ;
         const uint64_t fr = ~od & full & (~0ull << sp);
-        if (!fr) {  // find_free_memory found nothing: dropped
+        // (classify only gives destinations inside the grid; testing the cell
+        // number here makes its LDS read go out with the occupancy read)
+        if (!fr || dcn == 0xFFFFFFFFu) {  // find_free_memory found nothing: dropped
             ++n_drop;
             continue;
         }
         const int f = __builtin_ctzll(fr);
         sp = f;  // starting_point = free_idx
-        s_occ[n][tid] = od | (1ull << f);
+        const uint64_t fb = 1ull << f;
+        s_nb[n][tid].occ = od | fb;  // (the centre's LDS copy is not read)
         ++n_moved;
-        int64_t Dc = S;
-        if constexpr (kTab) {
-            Dc = s_dc[n][tid];
-        } else {
-            int r = n;
-            int64_t stride = 1;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                Dc += (int64_t)(r % 3 - 1) * stride;
-                r /= 3;
-                stride *= g.ni[d];
-            }
-        }
-        ms.org[Dc + C * f] = origin;
+        ms.org[org_at<D>(dcn, (uint32_t)f, C32, (uint32_t)K)] = origin;
         if (n == kCenter<D>) {
-            arrS |= 1ull << f;
-            selfins |= 1ull << f;
+            occC |= fb;
+            arrS |= fb;
+            selfins |= fb;
             if (f > ip) {  // the slot loop will reach this particle again
-                pend |= 1ull << f;
-                selfm |= 1ull << f;
+                pend |= fb;
+                selfm |= fb;
             }
         } else if (cd & kGap) {
             // the arrival fails the strict test of its new cell: when that cell's
             // phase comes later, it is re-slotted there (bisection returns that cell)
+            const int64_t Dc = dcn;
             int64_t J[D];
             unlin<D>(g.ni, Dc, J);
             if (phase_of<D>(J) > phase) {
-                ms.mov[Dc] |= 1ull << f;
+                ms.mov[Dc] |= fb;
                 ms.code[((f >> 3) * C + Dc) * 8 + (f & 7)] = (uint8_t)kCenter<D>;
             }
         }
     }
 #pragma unroll
     for (int n = 0; n < kNB<D>; ++n) {
-        if (n == kCenter<D> || nbc[n] < 0) continue;
-        const uint64_t now = s_occ[n][tid];
+        if (n == kCenter<D>) continue;
+        const NbEntry nb = s_nb[n][tid];
+        const uint32_t dn = nb.cell;
+        if (dn == 0xFFFFFFFFu) continue;
+        const uint64_t now = nb.occ;
         if (now != was[n]) {
             const uint64_t ins = now & ~was[n];
-            ms.occ[nbc[n]] = now;
+            ms.occ[dn] = now;
             if constexpr (kPre) {
-                ms.arr[nbc[n]] = arrN[n] | ins;
-                ms.tch[nbc[n]] = tchN[n] | ins;
+                ms.arr[dn] = arrN[n] | ins;
+                ms.tch[dn] = tchN[n] | ins;
             } else {
                 // no other thread of the phase touches these words: an atomic
                 // OR without return updates them with no load to wait for (a
                 // load here would also wait for every earlier store)
-                atomicOr(reinterpret_cast<unsigned long long*>(ms.arr + nbc[n]),
+                atomicOr(reinterpret_cast<unsigned long long*>(ms.arr + dn),
                          (unsigned long long)ins);
-                atomicOr(reinterpret_cast<unsigned long long*>(ms.tch + nbc[n]),
+                atomicOr(reinterpret_cast<unsigned long long*>(ms.tch + dn),
                          (unsigned long long)ins);
             }
         }
     }
-    ms.occ[S] = s_occ[kCenter<D>][tid];
+    ms.occ[S] = occC;
     ms.arr[S] = arrS;
     ms.tch[S] = tchS | selfins;
     ms.mov[S] = 0;
@@ -474,7 +478,10 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
         const uint32_t ab = (uint32_t)(arr >> k0) & 0xFFu;
         uint32_t o[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) o[j] = ((ab >> j) & 1) ? ms.org[c + C * (k0 + j)] : 0u;
+        for (int j = 0; j < 8; ++j)
+            o[j] = ((ab >> j) & 1) ? ms.org[org_at<D>((uint32_t)c, (uint32_t)(k0 + j), (uint32_t)C,
+                                                      (uint32_t)g.K)]
+                                   : 0u;
         T v[8][D];
 #pragma unroll
         for (int j = 0; j < 8; ++j)
@@ -807,7 +814,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     JP_LAUNCHED(ctx);
     int64_t total = 1;
     for (int d = 0; d < D; ++d) total *= (g.ni[d] + 2) / 3;
-    constexpr int BS = D == 2 ? 256 : 128;
+    constexpr int BS = D == 2 ? 128 : 64;  // measured: 256 and 64 (2D), 128 (3D) slower
     const int nphase = kNB<D>;
     for (int ph = 0; ph < nphase; ++ph) {
         Off3 o{{0, 0, 0}};
