@@ -201,19 +201,19 @@ __device__ inline unsigned xcd_block(unsigned w, unsigned n) {
 }
 
 // Active slots kb..ke-1 of cell c (bits at their slot positions), in rounds
-// of 16 clamped loads.
+// of 8 clamped loads (slot groups are multiples of 8).
 __device__ inline uint64_t active_slots_in(const uint8_t* __restrict__ index, int64_t c,
                                            int64_t C, int kb, int ke) {
     uint64_t bits = 0;
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        uint8_t m[16];
+    for (int k0 = kb; k0 < ke; k0 += 8) {
+        uint8_t m[8];
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
+        for (int v = 0; v < 8; ++v) {
             const int k = k0 + v < ke ? k0 + v : ke - 1;
             m[v] = index[c + C * k];
         }
 #pragma unroll
-        for (int v = 0; v < 16; ++v)
+        for (int v = 0; v < 8; ++v)
             if (k0 + v < ke) bits |= (uint64_t)(m[v] != 0) << (k0 + v);
     }
     return bits;

@@ -94,12 +94,13 @@ __device__ inline int phase_of(const int64_t* I) {
 }
 
 // ------------------------------------------------------------ 1. classify
-// Fields of the movers of one chunk of 8 slots, copied to the staging area in
-// one batch of independent loads (stg: chunk-local mover bits).
+// Fields a0.. of the movers of one chunk of 8 slots, copied to the staging
+// area in one batch of independent loads (stg: chunk-local mover bits).
 __device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint32_t stg,
-                                  int nargs, const ArgPtrs& args, const MoveScratch& ms) {
+                                  int nargs, const ArgPtrs& args, const MoveScratch& ms,
+                                  int a0 = 0) {
     if (!stg) return;
-    for (int a = 0; a < nargs; ++a) {
+    for (int a = a0; a < nargs; ++a) {
         const double* __restrict__ src = args.p[a];
         double* __restrict__ dst = ms.stga + a * S;
         double v[8];
@@ -467,21 +468,35 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
     if (c >= g.C) return;
     const int kb = (int)(lb % gridDim.y) * span;
     const int ke = kb + span < g.K ? kb + span : g.K;
+    const int64_t C = g.C, S = C * g.K;
+    // the first chunk's origins go out with the cell's words: unguarded loads
+    // (slot clamped to K - 1, values used for arrivals only), one round trip
+    // fewer than loading them after the arrival bits are known
+    uint32_t o0[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = kb + j < g.K ? kb + j : g.K - 1;
+        o0[j] = ms.org[org_at<D>((uint32_t)c, (uint32_t)k, (uint32_t)C, (uint32_t)g.K)];
+    }
     const uint64_t arr = ms.arr[c], occ = ms.occ[c], occ0 = ms.occ0[c], tch = ms.tch[c];
     const uint64_t work = arr | (~arr & ~occ & (occ0 | tch));
     if (!(work >> kb)) return;
-    const int64_t C = g.C, S = C * g.K;
     const T* __restrict__ sc = static_cast<const T*>(ms.stgc);
     for (int k0 = kb; k0 < ke; k0 += 8) {
         const uint32_t wb = (uint32_t)(work >> k0) & 0xFFu;
         if (!wb) continue;
         const uint32_t ab = (uint32_t)(arr >> k0) & 0xFFu;
         uint32_t o[8];
+        if (k0 == kb) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            o[j] = ((ab >> j) & 1) ? ms.org[org_at<D>((uint32_t)c, (uint32_t)(k0 + j), (uint32_t)C,
-                                                      (uint32_t)g.K)]
-                                   : 0u;
+            for (int j = 0; j < 8; ++j) o[j] = o0[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                o[j] = ((ab >> j) & 1) ? ms.org[org_at<D>((uint32_t)c, (uint32_t)(k0 + j),
+                                                          (uint32_t)C, (uint32_t)g.K)]
+                                       : 0u;
+        }
         T v[8][D];
 #pragma unroll
         for (int j = 0; j < 8; ++j)
@@ -691,13 +706,19 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
     float* __restrict__ sc = static_cast<float*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     constexpr int B = 8;
+    const double* __restrict__ f0 = nargs ? args.p[0] : nullptr;
     for (int k0 = kb; k0 < ke; k0 += B) {
+        // the offsets and the first field of the chunk's particles in one round
+        // (the field of a particle that stays is loaded for nothing; loading it
+        // after the movers are known would cost a round trip)
         float ub[B][D];
+        double fb[B];
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             const bool on = (occ >> (k0 + j)) & 1;
 #pragma unroll
             for (int d = 0; d < D; ++d) ub[j][d] = on ? off.p[d][c + C * (k0 + j)] : 0.0f;
+            fb[j] = (on && f0) ? f0[c + C * (k0 + j)] : 0.0;
         }
         uint64_t word = 0;
         uint32_t mv8 = 0, stg = 0;
@@ -739,7 +760,12 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args(c, C, S, k0, stg, nargs, args, ms);
+        if (stg && f0) {
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if ((stg >> j) & 1) ms.stga[c + C * (k0 + j)] = fb[j];
+        }
+        stage_args(c, C, S, k0, stg, nargs, args, ms, 1);
     }
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
 }
