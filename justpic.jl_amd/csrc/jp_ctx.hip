@@ -25,36 +25,64 @@ __global__ void __launch_bounds__(kBlock) k_occ_words(const uint8_t* __restrict_
     occ[c] = active_slots(index, c, C, K);
 }
 
-// Joins the per-wavefront list segments of one colour (blockIdx.x) into
-// list[q*cap + i], in wavefront order, and writes the colour's total.
-__global__ void __launch_bounds__(1024) k_join_segments(const int32_t* __restrict__ seg,
-                                                        const int32_t* __restrict__ cnt,
-                                                        int64_t nwave, int32_t* __restrict__ list,
-                                                        int32_t* __restrict__ total, int64_t cap) {
-    __shared__ int32_t sh[1024];
-    const int q = blockIdx.x, t = threadIdx.x;
-    const int64_t per = (nwave + 1023) / 1024;
-    const int64_t lo = t * per < nwave ? t * per : nwave;
-    const int64_t hi = lo + per < nwave ? lo + per : nwave;
+// Joins the per-wavefront list segments of colour q = blockIdx.y into
+// list[q*cap + i], in wavefront order, and writes the colour's total.  Each
+// workgroup owns kJoinSeg consecutive segments: it sums the counts of all
+// earlier segments (a few KB, read from L2) for its base offset, scans its own
+// counts in one wavefront, and copies with one wavefront per segment (lane i
+// moves entry i), all loads of a wavefront issued before its stores.
+constexpr int kJoinSeg = 64;
+constexpr int kJoinBlock = 256;
+__global__ void __launch_bounds__(kJoinBlock) k_join_segments(const int32_t* __restrict__ seg,
+                                                              const int32_t* __restrict__ cnt,
+                                                              int64_t nwave,
+                                                              int32_t* __restrict__ list,
+                                                              int32_t* __restrict__ total,
+                                                              int64_t cap) {
+    __shared__ int32_t s_part[kJoinBlock / 64];
+    __shared__ int32_t s_off[kJoinSeg], s_cnt[kJoinSeg];
+    const int q = blockIdx.y, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int32_t* __restrict__ cq = cnt + q * nwave;
-    int32_t s = 0;
-    for (int64_t w = lo; w < hi; ++w) s += cq[w];
-    sh[t] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan
-        const int32_t v = t >= o ? sh[t - o] : 0;
-        __syncthreads();
-        sh[t] += v;
-        __syncthreads();
+    const int64_t s0 = (int64_t)blockIdx.x * kJoinSeg;
+    const int64_t s1 = s0 + kJoinSeg < nwave ? s0 + kJoinSeg : nwave;
+    // base: counts of segments [0, s0)
+    int32_t acc = 0;
+#pragma unroll 8
+    for (int64_t w = t; w < s0; w += kJoinBlock) acc += cq[w];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    if (lane == 0) s_part[wv] = acc;
+    // this workgroup's counts, exclusive scan in wavefront 0
+    if (wv == 0) {
+        const int32_t n = s0 + lane < s1 ? cq[s0 + lane] : 0;
+        int32_t incl = n;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const int32_t v = __shfl_up(incl, m, 64);
+            if (lane >= m) incl += v;
+        }
+        s_cnt[lane] = n;
+        s_off[lane] = incl - n;
     }
-    int32_t off = sh[t] - s;
-    if (t == 1023) total[q] = sh[t];
+    __syncthreads();
+    int32_t base = 0;
+#pragma unroll
+    for (int i = 0; i < kJoinBlock / 64; ++i) base += s_part[i];
+    if (t == 0 && s1 == nwave) total[q] = base + s_off[kJoinSeg - 1] + s_cnt[kJoinSeg - 1];
+    // copy: wavefront wv moves segments s0 + wv, s0 + wv + 4, ...
+    constexpr int kPer = kJoinSeg / (kJoinBlock / 64);
     int32_t* __restrict__ lq = list + q * cap;
-    for (int64_t w = lo; w < hi; ++w) {
-        const int32_t n = cq[w];
-        const int32_t* __restrict__ sw = seg + (q * nwave + w) * 64;
-        for (int i = 0; i < n; ++i) lq[off + i] = sw[i];
-        off += n;
+    int32_t v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = wv + j * (kJoinBlock / 64);
+        const int64_t w = s0 + i;
+        v[j] = (w < s1 && lane < s_cnt[i]) ? seg[(q * nwave + w) * 64 + lane] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int i = wv + j * (kJoinBlock / 64);
+        if (s0 + i < s1 && lane < s_cnt[i]) lq[base + s_off[i] + lane] = v[j];
     }
 }
 
@@ -118,7 +146,10 @@ int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, 
 }
 
 int join_segments(jp_ctx* ctx, int nq, const InjectLists& L) {
-    k_join_segments<<<nq, 1024, 0, ctx->stream>>>(L.seg, L.cnt, L.nwave, L.list, L.total, L.cap);
+    const int64_t nb = (L.nwave + kJoinSeg - 1) / kJoinSeg;
+    const dim3 grid((unsigned)(nb > 0 ? nb : 1), (unsigned)nq);  // one block writes total = 0
+    k_join_segments<<<grid, kJoinBlock, 0, ctx->stream>>>(L.seg, L.cnt, L.nwave, L.list, L.total,
+                                                          L.cap);
     JP_LAUNCHED(ctx);
     return JP_OK;
 }
