@@ -45,8 +45,10 @@ struct MoveScratch {
     uint64_t* tch;   // slots that received a particle at some point
     uint8_t* code;   // per slot, 8 per word (word j*C + c): neighbour code / kDel, movers only
     uint32_t* org;   // per slot: original slot index of an arrival (layout: org_at)
-    void* stgc;      // D planes of C*K (coordinate type): staged mover coordinates
-    double* stga;    // nargs planes of C*K: staged mover fields, by original slot
+    char* stg;       // staged movers by original slot o = c + C k (layout: stg_coord)
+    int64_t S;       // slots
+    int64_t stride;  // 3D: record bytes (multiple of 8)
+    int64_t foff;    // 3D: byte offset of the fields in a record; 2D: of the field planes
 };
 
 // Position of (cell, slot) in the origin array.  2D: plane-major like the
@@ -93,22 +95,43 @@ __device__ inline int phase_of(const int64_t* I) {
     return ph;
 }
 
+// Staged coordinate d and field a of the mover from original slot o.  2D:
+// planes like the particle data (classify's stores coalesce across a
+// wavefront's cells).  3D: one record per slot, the D coordinates then the
+// fields, so that the scatter's gather of one mover touches one cache line
+// instead of one per array (measured: 3D move -7% split, -20% FP64; 2D +3%).
+template <int D, typename T>
+__device__ inline T& stg_coord(const MoveScratch& ms, int64_t o, int d) {
+    if constexpr (D == 3)
+        return reinterpret_cast<T*>(ms.stg + o * ms.stride)[d];
+    else
+        return reinterpret_cast<T*>(ms.stg)[d * ms.S + o];
+}
+template <int D>
+__device__ inline double& stg_field(const MoveScratch& ms, int64_t o, int a) {
+    if constexpr (D == 3)
+        return reinterpret_cast<double*>(ms.stg + o * ms.stride + ms.foff)[a];
+    else
+        return reinterpret_cast<double*>(ms.stg + ms.foff)[a * ms.S + o];
+}
+
 // ------------------------------------------------------------ 1. classify
 // Fields a0.. of the movers of one chunk of 8 slots, copied to the staging
 // area in one batch of independent loads (stg: chunk-local mover bits).
+template <int D>
 __device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint32_t stg,
                                   int nargs, const ArgPtrs& args, const MoveScratch& ms,
                                   int a0 = 0) {
     if (!stg) return;
+    (void)S;
     for (int a = a0; a < nargs; ++a) {
         const double* __restrict__ src = args.p[a];
-        double* __restrict__ dst = ms.stga + a * S;
         double v[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = ((stg >> j) & 1) ? src[c + C * (k0 + j)] : 0.0;
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            if ((stg >> j) & 1) dst[c + C * (k0 + j)] = v[j];
+            if ((stg >> j) & 1) stg_field<D>(ms, c + C * (k0 + j), a) = v[j];
     }
 }
 
@@ -177,7 +200,6 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
     const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
                                                 : active_slots_in(index, c, C, kb, ke);
     uint64_t mov = 0;
-    double* __restrict__ sc = static_cast<double*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
     // loads and stores across neighbouring cells)
@@ -237,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     cd = (uint8_t)n;
                     if (n != kCenter<D> && !inJ) cd |= kGap;
 #pragma unroll
-                    for (int d = 0; d < D; ++d) sc[d * S + c + C * k] = p[d];
+                    for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, c + C * k, d) = p[d];
                     stg |= 1u << j;
                 }
             }
@@ -245,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args(c, C, S, k0, stg, nargs, args, ms);
+        stage_args<D>(c, C, S, k0, stg, nargs, args, ms);
     }
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
 }
@@ -481,7 +503,6 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
     const uint64_t arr = ms.arr[c], occ = ms.occ[c], occ0 = ms.occ0[c], tch = ms.tch[c];
     const uint64_t work = arr | (~arr & ~occ & (occ0 | tch));
     if (!(work >> kb)) return;
-    const T* __restrict__ sc = static_cast<const T*>(ms.stgc);
     for (int k0 = kb; k0 < ke; k0 += 8) {
         const uint32_t wb = (uint32_t)(work >> k0) & 0xFFu;
         if (!wb) continue;
@@ -502,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
         for (int j = 0; j < 8; ++j)
 #pragma unroll
             for (int d = 0; d < D; ++d)
-                v[j][d] = ((ab >> j) & 1) ? sc[d * S + o[j]] : (T)dnan();
+                v[j][d] = ((ab >> j) & 1) ? stg_coord<D, T>(ms, o[j], d) : (T)dnan();
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             if (!((wb >> j) & 1)) continue;
@@ -514,7 +535,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
         for (int a = 0; a < nargs; ++a) {
             double av[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) av[j] = ((ab >> j) & 1) ? ms.stga[a * S + o[j]] : dnan();
+            for (int j = 0; j < 8; ++j) av[j] = ((ab >> j) & 1) ? stg_field<D>(ms, o[j], a) : dnan();
 #pragma unroll
             for (int j = 0; j < 8; ++j)
                 if ((wb >> j) & 1) args.p[a][c + C * (k0 + j)] = av[j];
@@ -703,7 +724,6 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
     const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
                                                 : active_slots_in(index, c, C, kb, ke);
     uint64_t mov = 0;
-    float* __restrict__ sc = static_cast<float*>(ms.stgc);
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     constexpr int B = 8;
     const double* __restrict__ f0 = nargs ? args.p[0] : nullptr;
@@ -753,7 +773,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 cd = (uint8_t)n;
 #pragma unroll
                 for (int d = 0; d < D; ++d)
-                    sc[d * S + c + C * k] = (float)((double)ub[j][d] - (double)s[d]);
+                    stg_coord<D, float>(ms, c + C * k, d) = (float)((double)ub[j][d] - (double)s[d]);
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
@@ -763,9 +783,9 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
         if (stg && f0) {
 #pragma unroll
             for (int j = 0; j < B; ++j)
-                if ((stg >> j) & 1) ms.stga[c + C * (k0 + j)] = fb[j];
+                if ((stg >> j) & 1) stg_field<D>(ms, c + C * (k0 + j), 0) = fb[j];
         }
-        stage_args(c, C, S, k0, stg, nargs, args, ms, 1);
+        stage_args<D>(c, C, S, k0, stg, nargs, args, ms, 1);
     }
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
 }
@@ -802,8 +822,17 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     const size_t o_tch = off;  off = align_up(off + C * 8, 256);
     const size_t o_code = off; off = align_up(off + (size_t)C * 64, 256);
     const size_t o_org = off;  off = align_up(off + (size_t)S * 4, 256);
-    const size_t o_stgc = off; off = align_up(off + (size_t)S * sizeof(T) * D, 256);
-    const size_t o_stga = off; off = align_up(off + (size_t)S * 8 * nargs, 256);
+    ms.S = S;
+    if (D == 3) {
+        ms.foff = (int64_t)align_up(sizeof(T) * D, 8);
+        ms.stride = ms.foff + 8 * nargs;
+    } else {
+        ms.foff = (int64_t)align_up((size_t)S * sizeof(T) * D, 256);
+        ms.stride = 0;
+    }
+    const size_t stg_bytes = D == 3 ? (size_t)S * (size_t)ms.stride
+                                    : (size_t)ms.foff + (size_t)S * 8 * nargs;
+    const size_t o_stg = off;  off = align_up(off + stg_bytes, 256);
     int rc = ensure_scratch(ctx, off);
     if (rc) return rc;
     char* base = static_cast<char*>(ctx->scratch);
@@ -814,8 +843,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     ms.tch = (uint64_t*)(base + o_tch);
     ms.code = (uint8_t*)(base + o_code);
     ms.org = (uint32_t*)(base + o_org);
-    ms.stgc = (void*)(base + o_stgc);
-    ms.stga = (double*)(base + o_stga);
+    ms.stg = base + o_stg;
 
     int* flag = dflags(ctx);
     if ((rc = zero_bytes(ctx, flag, sizeof(int)))) return rc;
