@@ -49,6 +49,7 @@ struct MoveScratch {
     int64_t S;       // slots
     int64_t stride;  // 3D: record bytes (multiple of 8)
     int64_t foff;    // 3D: byte offset of the fields in a record; 2D: of the field planes
+                     // (after the interleaved coordinates)
 };
 
 // Position of (cell, slot) in the origin array.  2D: plane-major like the
@@ -95,17 +96,18 @@ __device__ inline int phase_of(const int64_t* I) {
     return ph;
 }
 
-// Staged coordinate d and field a of the mover from original slot o.  2D:
-// planes like the particle data (classify's stores coalesce across a
-// wavefront's cells).  3D: one record per slot, the D coordinates then the
-// fields, so that the scatter's gather of one mover touches one cache line
-// instead of one per array (measured: 3D move -7% split, -20% FP64; 2D +3%).
+// Staged coordinate d and field a of the mover from original slot o.  The
+// scatter gathers movers by origin, one cache line per array touched, so the
+// arrays are few.  2D: the D coordinates interleaved in one array, fields in
+// planes (classify's stores stay contiguous across a wavefront's cells).  3D:
+// one record per slot, the D coordinates then the fields (3D move -7% split,
+// -20% FP64 against planes; full records were 3% slower in 2D).
 template <int D, typename T>
 __device__ inline T& stg_coord(const MoveScratch& ms, int64_t o, int d) {
     if constexpr (D == 3)
         return reinterpret_cast<T*>(ms.stg + o * ms.stride)[d];
     else
-        return reinterpret_cast<T*>(ms.stg)[d * ms.S + o];
+        return reinterpret_cast<T*>(ms.stg)[o * D + d];
 }
 template <int D>
 __device__ inline double& stg_field(const MoveScratch& ms, int64_t o, int a) {
