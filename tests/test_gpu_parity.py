@@ -158,6 +158,36 @@ def test_move_parity(ctx, D):
         assert_field(Td.data.flat(), T, exact=True)
 
 
+@pytest.mark.parametrize("D,K,npq", [(2, 64, 12), (2, 54, 10), (3, 64, 6), (3, 54, 5)])
+def test_move_parity_wide_K(ctx, D, K, npq):
+    """Wide cells: K = 64 (every bit of the 64-bit occupancy words) and a K
+    that is not a multiple of 8, with crowded cells, through slot groups and
+    their per-cell byte stores."""
+    say("move wide K", D, K)
+    if D == 2:
+        xv, xi = grid2d(20, 18)
+        V = stream_V(xi)
+    else:
+        xv, xi = grid3d(7)
+        V = cellular_V(xi)
+    hp = host_particles(xv, xi, K, npq, seed=29)
+    T = np.where(hp.index == 1, np.arange(hp.S, dtype=np.float64), np.nan)
+    p = to_dev(ctx, hp)
+    Td = jp.CellArray(ctx, hp.ni, hp.K)
+    Td.copy_from(T)
+    Vd = dev_V(ctx, V)
+    dt = cfl_dt(xv, V)
+    for it in range(3):
+        jp.advection_(p, jp.RungeKutta2(), Vd, dt)
+        O.advection(hp, V, O.JP_RK2, 0.5, dt)
+        st_d = jp.move_particles_(p, (Td,), stats=True)
+        st_o = O.move_particles(hp, [T])
+        assert st_d == st_o
+        assert st_o["moved"] > 0
+        assert_state_equal(p, hp, exact=True)
+        assert_field(Td.data.flat(), T, exact=True)
+
+
 def test_move_small_K_drops_and_deletes(ctx):
     """Crowded cells (K small) force drops; a large dt pushes particles out of
     the domain (deletions)."""
