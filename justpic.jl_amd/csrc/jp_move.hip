@@ -7,17 +7,18 @@
 // are occupied, so it is computed on 64-bit occupancy words and the particle
 // data are moved afterwards in one parallel pass:
 //
-//   1. classify (one thread per cell, all cells, coalesced over the cell
-//      planes): in-cell test, domain test, destination (bisection seeded at
-//      the cell) as a neighbour code, occupancy/mover words, and a copy of
-//      every mover's coordinates and fields into a staging area indexed by
-//      its original slot;
+//   1. classify (one thread per cell, or per cell and group of slots when the
+//      cells alone do not fill the chip; coalesced over the cell planes):
+//      in-cell test, domain test, destination (bisection seeded at the cell)
+//      as a neighbour code, occupancy/mover words, and a copy of every
+//      mover's coordinates and fields into a staging area indexed by its
+//      original slot;
 //   2. 3^D colour phases (one thread per source cell of the colour): the
 //      reference's slot loop replayed on the occupancy words of the 3^D
 //      neighbourhood (kept in LDS), recording for every slot that received a
 //      particle where that particle came from;
-//   3. scatter (one thread per cell): arrivals copy their staged data into
-//      their final slot, vacated slots get NaN and mask false.
+//   3. scatter (threads as in classify): arrivals copy their staged data
+//      into their final slot, vacated slots get NaN and mask false.
 //
 // A particle that jumps more than one cell breaks the colour independence;
 // classify raises a flag and a single-thread kernel then runs the reference
