@@ -10,7 +10,8 @@ import pytest
 import justpic_hip as jp
 from oracle import oracle as O
 
-from helpers import cellular_V, cfl_dt, clear, grid2d, grid3d, host_particles, place, stream_V
+from helpers import (cellular_V, cfl_dt, clear, elem, grid2d, grid3d, host_particles, place,
+                     stream_V)
 
 pytestmark = pytest.mark.gpu
 
@@ -211,6 +212,40 @@ def test_move_small_K_drops_and_deletes(ctx):
         assert_state_equal(p, hp, exact=True)
     say("move drops totals", tot)
     assert tot["dropped"] > 0 and tot["deleted"] > 0
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_move_vertex_ties(ctx, D):
+    """Particles exactly on vertex lines (h = 0.125, exact in binary).  From
+    cell 1, a particle on the right vertex of cell 2 goes to cell 2, fails its
+    strict in-cell test, and is re-slotted there when cell 2's colour phase
+    comes later (an arrival that moves on: its origin is read back).  From
+    cell 2, one on the right vertex of cell 3 stays in cell 3, whose phase has
+    already run; the starting point carried from the re-slot puts it in slot 1.
+    A particle on its own cell's left vertex is re-slotted in that cell.
+    Compared with the oracle bit for bit, the field included."""
+    say("move vertex ties", D)
+    xv, xi = grid2d(8) if D == 2 else grid3d(8)
+    hp = O.HostParticles(xv, 6, xi)
+    clear(hp)
+    T = np.full(hp.S, np.nan)
+    t1, t2, y = (1,) * (D - 1), (2,) * (D - 1), (0.2,) * (D - 1)
+    place(hp, (1,) + t1, 0, (0.375,) + y, [T], [1.0])
+    place(hp, (1,) + t1, 1, (0.2,) + y, [T], [2.0])
+    place(hp, (2,) + t1, 0, (0.3,) + y, [T], [3.0])
+    place(hp, (2,) + t1, 2, (0.5,) + y, [T], [4.0])
+    place(hp, (2,) + t2, 1, (0.25,) + (0.3,) * (D - 1), [T], [5.0])
+    p = to_dev(ctx, hp)
+    Td = jp.CellArray(ctx, hp.ni, hp.K)
+    Td.copy_from(T)
+    st_d = jp.move_particles_(p, (Td,), stats=True)
+    st_o = O.move_particles(hp, [T])
+    say("vertex tie stats", st_d, st_o)
+    assert st_d == st_o and st_o["moved"] == 4
+    assert_state_equal(p, hp, exact=True)
+    assert_field(Td.data.flat(), T, exact=True)
+    # the arrival re-slotted in cell 2 and the one kept in cell 3
+    assert T[elem(hp, (2,) + t1, 1)] == 1.0 and T[elem(hp, (3,) + t1, 1)] == 4.0
 
 
 def test_move_far_jumps_serial_path(ctx):
