@@ -21,8 +21,9 @@
 //      into their final slot, vacated slots get NaN and mask false.
 //
 // A particle that jumps more than one cell breaks the colour independence;
-// classify raises a flag and a single-thread kernel then runs the reference
-// loop for every phase in order (the other kernels return immediately).
+// classify raises a flag, the phases and the scatter return immediately, and
+// a single-thread kernel, launched last, runs the reference loop for every
+// phase in order and clears the flag.
 #include "jp_ctx.hpp"
 
 #include <type_traits>
@@ -673,7 +674,7 @@ __device__ void move_cell_serial(const Grid<D>& g, const TPtr3<D, float>& off, u
 
 template <int D, typename T>
 __global__ void k_mv_serial(Grid<D> g, TPtr3<D, T> coords, uint8_t* index, int nargs,
-                            ArgPtrs args, const int* __restrict__ flag, jp_stats* st) {
+                            ArgPtrs args, int* __restrict__ flag, jp_stats* st) {
     if (!*flag || blockIdx.x != 0 || threadIdx.x != 0) return;
     int64_t ncol[D];
     int64_t total = 1;
@@ -705,6 +706,7 @@ __global__ void k_mv_serial(Grid<D> g, TPtr3<D, T> coords, uint8_t* index, int n
     stat_add(st, ST_DELETED, cnt[1]);
     stat_add(st, ST_DROPPED, cnt[2]);
     stat_add(st, ST_FAR, cnt[3]);
+    *flag = 0;  // ready for the next move on this context
 }
 
 // ------------------------------------------------------------ split classify
@@ -808,7 +810,9 @@ TPtr3<D, T> tptr3(T* const* c) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// classify -> 3^D colour phases -> serial fallback -> scatter
+// classify -> 3^D colour phases -> scatter -> serial fallback (after a far
+// jump the phases and the scatter return at once, and the serial kernel moves
+// everything from the untouched data)
 template <int D, typename T>
 int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* index, int nargs,
              double* const* args, jp_stats* stats) {
@@ -849,7 +853,8 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     ms.stg = base + o_stg;
 
     int* flag = dflags(ctx);
-    if ((rc = zero_bytes(ctx, flag, sizeof(int)))) return rc;
+    // the far-jump flag is zero here: a new context starts with it cleared,
+    // and the serial fallback, the last kernel of a move, clears it after use
     if ((rc = stats_begin(ctx, stats))) return rc;
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     ArgPtrs ap = argptrs(args, nargs);
@@ -884,10 +889,10 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
                                                                                     flag, st);
         JP_LAUNCHED(ctx);
     }
-    k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
-    JP_LAUNCHED(ctx);
     k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, flag,
                                                          span);
+    JP_LAUNCHED(ctx);
+    k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
     JP_LAUNCHED(ctx);
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);
