@@ -89,6 +89,21 @@ int jp_ctx_set_sync(jp_ctx* ctx, int synchronous);
 /* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
 int jp_ctx_set_stream(jp_ctx* ctx, void* hip_stream);
 void* jp_ctx_stream(jp_ctx* ctx);
+/* Launch options.  Results never depend on them; they select launch shapes
+ * that the library otherwise chooses from the problem size, so that small
+ * problems can run the shapes large ones use (the tests compare every
+ * setting with the oracle).
+ *   JP_OPT_SLOT_SPAN    slots per slot group of the per-cell kernels (move
+ *                       classify and scatter, FP32-split advection): 0
+ *                       (default) chooses from the grid size; otherwise a
+ *                       multiple of 8 up to 64 (>= max_xcell: whole cells).
+ *   JP_OPT_INJECT_WAVES persistent wavefronts per injection colour launch: 0
+ *                       (default) is one per listed cell up to 4096.
+ * No reference counterpart (the KernelAbstractions launch sizes of the
+ * reference are chosen by its backend). */
+#define JP_OPT_SLOT_SPAN 1
+#define JP_OPT_INJECT_WAVES 2
+int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value);
 int jp_sync(jp_ctx* ctx);
 int jp_malloc(jp_ctx* ctx, void** dptr, size_t bytes);
 int jp_free(jp_ctx* ctx, void* dptr);

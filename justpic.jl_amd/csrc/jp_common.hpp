@@ -50,6 +50,27 @@ struct Off3 {
     int o[3];
 };
 
+// Geometry of the FP32-split form, written by the geometry kernel at the start
+// of every split entry point that needs it and read by that call's kernels.
+// A position is the cell I of its slot plus an offset u in cell units of the
+// uniform vertex axes.  Velocity component c is sampled on a uniform axis
+// along each dimension d whose node coordinate at that position is
+// (I + u) r[c][d] + o[c][d]: r = vertex spacing / node spacing, o = (first
+// vertex - first node) / node spacing.  The standard staggering has r = 1 and
+// o = 0 (vertex axis) or 1/2 (centre axis with one ghost node per side); the
+// README's expand_range rounds the ghost ends to 5 significant digits
+// (README.md:48-55), which moves r and o slightly away from those values.
+struct SplitGeom {
+    int32_t bad;     // this call: 0 accepted, bit 0 vertex axis, bit 1 velocity axis
+    int32_t sticky;  // OR of `bad` since the last report (asynchronous calls)
+    double r[3][3];
+    double o[3][3];
+};
+constexpr int kGeomVertex = 1;
+constexpr int kGeomVelocity = 2;
+// accepted deviation of an axis from x[0] + k h, relative to h
+constexpr double kGeomTol = 1.0e-9;
+
 // 0-based cell tuple <-> column-major linear index
 template <int D>
 __host__ __device__ inline int64_t lin(const int64_t (&ni)[D], const int64_t* I) {

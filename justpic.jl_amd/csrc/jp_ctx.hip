@@ -265,10 +265,29 @@ int jp_ctx_set_stream(jp_ctx* ctx, void* s) {
 
 void* jp_ctx_stream(jp_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value) {
+    if (!ctx) return JP_ERR_STATE;
+    if (ctx->capturing) return fail(ctx, JP_ERR_STATE, "cannot change options while capturing%s");
+    switch (option) {
+    case JP_OPT_SLOT_SPAN:
+        if (value < 0 || value > 64 || value % 8)
+            return fail(ctx, JP_ERR_ARG, "slot span must be 0 or a multiple of 8 up to 64%s");
+        ctx->opt_span = (int)value;
+        return JP_OK;
+    case JP_OPT_INJECT_WAVES:
+        if (value < 0 || value > (1 << 20))
+            return fail(ctx, JP_ERR_ARG, "injection wavefronts must be in 0..2^20%s");
+        ctx->opt_inject_waves = value;
+        return JP_OK;
+    default:
+        return fail(ctx, JP_ERR_ARG, "unknown option%s %lld", "", option);
+    }
+}
+
 int jp_sync(jp_ctx* ctx) {
     if (!ctx) return JP_ERR_STATE;
     JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return JP_OK;
+    return ctx->capturing ? JP_OK : geom_report(ctx);
 }
 
 int jp_malloc(jp_ctx* ctx, void** p, size_t bytes) {
@@ -298,7 +317,7 @@ int jp_memcpy_d2h(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return JP_OK;
     JP_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return JP_OK;
+    return ctx->capturing ? JP_OK : geom_report(ctx);
 }
 
 int jp_memcpy_d2d(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {

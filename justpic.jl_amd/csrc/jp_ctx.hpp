@@ -21,6 +21,9 @@ struct jp_ctx {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     int num_cu = 256;
+    // launch options (jp_ctx_set_option); 0 = chosen from the problem size
+    int opt_span = 0;
+    int64_t opt_inject_waves = 0;
 };
 
 namespace jp {
@@ -48,6 +51,20 @@ inline jp_stats* dstats(jp_ctx* ctx) { return reinterpret_cast<jp_stats*>(ctx->d
 inline int* dflags(jp_ctx* ctx) {
     return reinterpret_cast<int*>(static_cast<char*>(ctx->dsmall) + 256);
 }
+inline SplitGeom* dgeom(jp_ctx* ctx) {
+    return reinterpret_cast<SplitGeom*>(static_cast<char*>(ctx->dsmall) + 1024);
+}
+
+// FP32-split entry points: launch the geometry check of the vertex axes (and,
+// when vg is not null, the affine maps of the velocity axes) into dgeom(ctx).
+// The call's kernels skip all work when it fails.
+int split_geometry(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg);
+// finish() of a split entry point: a synchronous call reports a rejected
+// grid as JP_ERR_ARG; an asynchronous one leaves it in the sticky word, which
+// the next jp_sync or jp_memcpy_d2h reports.
+int split_finish(jp_ctx* ctx);
+// report (and clear) a rejection left by asynchronous split calls
+int geom_report(jp_ctx* ctx);
 
 // scratch of at least `bytes`; grows outside capture only
 int ensure_scratch(jp_ctx* ctx, size_t bytes);
@@ -65,9 +82,10 @@ struct InjectLists {
 };
 int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, InjectLists& L);
 int join_segments(jp_ctx* ctx, int nq, const InjectLists& L);
-// persistent wavefronts for a work list of at most n entries
-inline int64_t inject_blocks(int64_t n) {
-    const int64_t w = n < 4096 ? n : 4096;
+// workgroups (4 wavefronts each) of persistent wavefronts for a work list of
+// at most n entries: one wavefront per entry up to 4096
+inline int64_t inject_blocks(const jp_ctx* ctx, int64_t n) {
+    const int64_t w = ctx->opt_inject_waves > 0 ? ctx->opt_inject_waves : (n < 4096 ? n : 4096);
     return (w + 3) / 4 > 0 ? (w + 3) / 4 : 1;
 }
 // occupancy words (bit k = slot k active) of the C cells, one kernel
@@ -85,7 +103,9 @@ inline int64_t blocks_for(int64_t n, int b = kBlock) { return (n + b - 1) / b; }
 // workgroups over K slots: groups are added (blockIdx.y) until the launch has
 // about `target` workgroups (default 8 per CU of the 256), at most one group
 // per 8 slots.
-inline int slot_span(int64_t blocks, int K, int64_t target = 2048) {
+// JP_OPT_SLOT_SPAN, when set, replaces the choice.
+inline int slot_span(const jp_ctx* ctx, int64_t blocks, int K, int64_t target = 2048) {
+    if (ctx->opt_span > 0) return ctx->opt_span;
     const int64_t chunks = (K + 7) / 8;
     int64_t g = (target + blocks - 1) / (blocks > 0 ? blocks : 1);
     g = g < 1 ? 1 : (g > chunks ? chunks : g);

@@ -761,7 +761,7 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
         // persistent wavefronts, one listed cell at a time each
         if constexpr (D == 2) {
             if (nargs <= 1) {
-                k_inject_wave2<<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+                k_inject_wave2<<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
                     G, cp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
                     L.list + (int64_t)ph * cap, L.total + ph, L.occ,
                     static_cast<double*>(L.box), st);
@@ -769,7 +769,7 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
                 continue;
             }
         }
-        k_inject_wave<D><<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+        k_inject_wave<D><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
             G, cp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
             L.total + ph, st);
         JP_LAUNCHED(ctx);

@@ -23,7 +23,9 @@
 // A particle that jumps more than one cell breaks the colour independence;
 // classify raises a flag, the phases and the scatter return immediately, and
 // a single-thread kernel, launched last, runs the reference loop for every
-// phase in order and clears the flag.
+// phase in order and clears the flag.  The split classify raises another bit
+// of the same flag when the grid's geometry was rejected: then nothing moves
+// and the last kernel only clears the flag.
 #include "jp_ctx.hpp"
 
 #include <type_traits>
@@ -33,6 +35,8 @@ using namespace jp;
 namespace {
 
 constexpr uint8_t kDel = 0xFE;
+constexpr int kFlagFar = 1;       // a mover jumps more than one cell
+constexpr int kFlagRejected = 2;  // split geometry rejected
 constexpr uint8_t kGap = 0x80;  // arrival fails the strict in-cell test of its destination
 // s_waitcnt immediate of gfx9 for vmcnt(0) with expcnt and lgkmcnt left free
 constexpr int kVmcnt0 = 0x0F70;
@@ -257,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     inJ = inJ & (lo < p[d] && p[d] < lo + h);
                 }
                 if (far) {
-                    atomicOr(flag, 1);
+                    atomicOr(flag, kFlagFar);
                     cd = kDel;  // unused: the serial kernel takes over
                 } else {
                     cd = (uint8_t)n;
@@ -676,6 +680,10 @@ template <int D, typename T>
 __global__ void k_mv_serial(Grid<D> g, TPtr3<D, T> coords, uint8_t* index, int nargs,
                             ArgPtrs args, int* __restrict__ flag, jp_stats* st) {
     if (!*flag || blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (*flag & kFlagRejected) {  // split grid rejected: nothing moves
+        *flag = 0;
+        return;
+    }
     int64_t ncol[D];
     int64_t total = 1;
     for (int d = 0; d < D; ++d) {
@@ -718,8 +726,13 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D, float> off,
                                                               const uint8_t* __restrict__ index,
                                                               int nargs, ArgPtrs args,
-                                                              MoveScratch ms, int* flag, int span) {
+                                                              MoveScratch ms, int* flag, int span,
+                                                              const SplitGeom* __restrict__ geo) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (geo->bad) {  // grid rejected: the later kernels of this move return at once
+        if (c == 0 && blockIdx.y == 0) atomicOr(flag, kFlagRejected);
+        return;
+    }
     if (c >= g.C) return;
     const int kb = blockIdx.y * span;  // slot group kb..ke-1
     const int ke = kb + span < g.K ? kb + span : g.K;
@@ -772,7 +785,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
             if (out) {
                 cd = kDel;
             } else if (far) {
-                atomicOr(flag, 1);
+                atomicOr(flag, kFlagFar);
                 cd = kDel;  // unused: the serial kernel takes over
             } else {
                 cd = (uint8_t)n;
@@ -862,7 +875,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     // per-cell kernels: slot groups on blockIdx.y when the cells alone do not
     // fill the chip (up to 24 workgroups per CU: with one 8-slot chunk per
     // thread, no chunk's loads wait for an earlier chunk's stores)
-    const int span = slot_span(blocks_for(C), g.K, 6144);
+    const int span = slot_span(ctx, blocks_for(C), g.K, 6144);
     const dim3 cgrid((unsigned)blocks_for(C), (unsigned)((g.K + span - 1) / span));
     if constexpr (std::is_same<T, double>::value) {
         CPtr3 cc{{nullptr, nullptr, nullptr}};
@@ -870,8 +883,9 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
         k_mv_classify<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms, flag,
                                                            span);
     } else {
+        if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
         k_mv_classify_split<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
-                                                                 flag, span);
+                                                                 flag, span, dgeom(ctx));
     }
     JP_LAUNCHED(ctx);
     int64_t total = 1;
@@ -895,7 +909,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
     JP_LAUNCHED(ctx);
     if ((rc = stats_end(ctx, stats))) return rc;
-    return finish(ctx);
+    return std::is_same<T, double>::value ? finish(ctx) : split_finish(ctx);
 }
 
 int check_ptrs(jp_ctx* ctx, int n, const void* const* p, const char* what) {

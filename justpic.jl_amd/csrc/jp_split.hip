@@ -46,6 +46,75 @@ SGrid<D> sgrid(const Grid<D>& g) {
 __device__ inline float fnan() { return __builtin_nanf(""); }
 __device__ inline double dnan() { return __builtin_nan(""); }
 
+// ------------------------------------------------------------ geometry
+// Deviation of an axis from uniform spacing, relative to its mean spacing h =
+// (x[n-1] - x[0]) / (n-1): max over k of |x[k] - (x[0] + k h)| and, with
+// with_dx, of |dx[k] - h| (the spacing the reference stores in di).  A NaN or
+// a non-increasing axis gives +Inf.  All threads of the workgroup call it.
+__device__ double axis_deviation(const Axis& a, bool with_dx, double& h, double* red) {
+    const double x0 = a.x[0], xn = a.x[a.n - 1];
+    h = (xn - x0) / (double)(a.n - 1);
+    double m = 0.0;
+    for (int64_t k = threadIdx.x; k < a.n; k += blockDim.x) {
+        const double e = fabs(a.x[k] - fma((double)k, h, x0));
+        m = e == e ? fmax(m, e) : __builtin_inf();
+    }
+    if (with_dx)
+        for (int64_t k = threadIdx.x; k + 1 < a.n; k += blockDim.x) {
+            const double e = fabs(a.dx[k] - h);
+            m = e == e ? fmax(m, e) : __builtin_inf();
+        }
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, red[w]);
+    __syncthreads();
+    return (h > 0.0 && h < __builtin_inf()) ? r / h : __builtin_inf();
+}
+
+// One workgroup: checks that the vertex axes are uniform (split offsets are in
+// cell units of one spacing per dimension) and, with velocity axes, that each
+// of them is uniform, and derives their affine maps (SplitGeom).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_split_geom(Grid<D> g, Vel<D> v, int with_vel,
+                                                       SplitGeom* __restrict__ geo) {
+    __shared__ double red[kBlock / 64];
+    int bad = 0;
+    double h[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (!(axis_deviation(g.xv[d], true, h[d], red) <= kGeomTol)) bad |= kGeomVertex;
+    double r[D][D], o[D][D];
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            r[c][d] = 1.0;
+            o[c][d] = 0.0;
+            if (!with_vel) continue;
+            double hv;
+            if (!(axis_deviation(v.ax[c][d], false, hv, red) <= kGeomTol)) {
+                bad |= kGeomVelocity;
+                continue;
+            }
+            r[c][d] = h[d] / hv;
+            o[c][d] = (g.xv[d].x[0] - v.ax[c][d].x[0]) / hv;
+        }
+    if (threadIdx.x == 0) {
+        geo->bad = bad;
+        geo->sticky |= bad;
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                geo->r[c][d] = r[c][d];
+                geo->o[c][d] = o[c][d];
+            }
+    }
+}
+
 // ------------------------------------------------------------ conversions
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_to_split(Grid<D> g, CPtr3 x, const uint8_t* index,
@@ -114,39 +183,81 @@ struct SVelIn {
     int n[D][D];
 };
 
+// Affine maps of one grid's velocity axes (SplitGeom::r, o)
+template <int D>
+struct SMap {
+    double r[D][D], o[D][D];
+};
+
+// Per cell I and velocity axis (c, d): the node coordinate of the point I + u
+// is I + ei + (u r + ef), where e = I (r - 1) + o = ei + ef is split in FP64
+// into an integer and a fraction once per cell, so that the float32
+// arithmetic per stage only handles values of order one.
+template <int D>
+struct CellMap {
+    int ei[D][D];
+    float ef[D][D];
+    float r[D][D];
+};
+
+template <int D>
+__device__ inline void cell_map(const SMap<D>& m, const int (&I)[D], CellMap<D>& cm) {
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const double e = fma((double)I[d], m.r[c][d] - 1.0, m.o[c][d]);
+            const double fl = floor(e);
+            cm.ei[c][d] = (int)fl;
+            cm.ef[c][d] = (float)(e - fl);
+            cm.r[c][d] = (float)m.r[c][d];
+        }
+}
+
+// first node of the LDS window of component c along d for a tile at t0
+template <int D>
+__device__ inline int window_origin(const SMap<D>& m, int c, int d, int t0) {
+    return t0 + (int)floor(fma((double)t0, m.r[c][d] - 1.0, m.o[c][d])) - 1;
+}
+
 // velocity of component c at cell I + offset u (u in cell units, any value),
-// from the LDS window sw (origin t0 - 1) when it holds the four/eight corners
+// from the LDS window sw (first node w0 per dimension) when it holds the
+// four/eight corners, else from global memory.  Outside the limits of the
+// component's grid the result is +Inf, as in check_local_limits
+// (src/Advection/advection.jl:39-47: first <= p <= last, inclusive).
 template <int D, class TS>
-__device__ inline float interp_tile(const float* __restrict__ sw, const SVelIn<D>& V,
-                                    const SGrid<D>& g, int c, double sc, const int (&t0)[D],
-                                    const int (&I)[D], const float (&u)[D]) {
+__device__ inline float interp_tile(const float* __restrict__ sw, const SVelIn<D>& V, int c,
+                                    double sc, const int (&w0)[D], const int (&I)[D],
+                                    const CellMap<D>& cm, const float (&u)[D]) {
     int k[D];
     float t[D];
     bool inw = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         if (!(fabsf(u[d]) < 1.0e6f)) return __builtin_inff();
-        const float s = floorf(u[d]);
-        const int J = I[d] + (int)s;
-        const float f = u[d] - s;
-        if (d == c) {  // vertex axis of the component: limits [0, ni] cells
-            if (J < 0 || J >= g.ni[d]) return __builtin_inff();
-            k[d] = J;
-            t[d] = f;
-        } else {  // centre axis with ghosts: limits [-1/2, ni + 1/2] cells
-            const bool up = f >= 0.5f;
-            k[d] = J + (up ? 1 : 0);
-            if (k[d] < 0 || k[d] > g.ni[d]) return __builtin_inff();
-            t[d] = up ? f - 0.5f : f + 0.5f;
+        const float s = fmaf(u[d], cm.r[c][d], cm.ef[c][d]);
+        const float fl = floorf(s);
+        int kk = I[d] + cm.ei[c][d] + (int)fl;
+        float tt = s - fl;
+        const int n = V.n[c][d];
+        if (kk < 0 || kk > n - 2) {
+            if (kk == n - 1 && tt == 0.0f) {  // on the last node
+                kk = n - 2;
+                tt = 1.0f;
+            } else {
+                return __builtin_inff();
+            }
         }
-        const int kl = k[d] - (t0[d] - 1);
+        k[d] = kk;
+        t[d] = tt;
+        const int kl = kk - w0[d];
         inw = inw && kl >= 0 && kl + 1 < TS::w(d);
     }
     float q[1 << D];
     if (inw) {
         constexpr int W0 = TS::w(0), W1 = TS::w(1);
-        int b = (k[0] - t0[0] + 1) + W0 * (k[1] - t0[1] + 1);
-        if constexpr (D == 3) b += W0 * W1 * (k[2] - t0[2] + 1);
+        int b = (k[0] - w0[0]) + W0 * (k[1] - w0[1]);
+        if constexpr (D == 3) b += W0 * W1 * (k[2] - w0[2]);
 #pragma unroll
         for (int r = 0; r < (1 << D); ++r) {
             const int o = (r & 1) + ((r >> 1) & 1) * W0 + (D == 3 ? ((r >> 2) & 1) * W0 * W1 : 0);
@@ -180,12 +291,12 @@ struct RKf {
 
 template <int D, int M, class TS>
 __device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVelIn<D>& V,
-                                        const SGrid<D>& g, const double (&sc)[D],
-                                        const int (&t0)[D], const int (&I)[D], const RKf& P,
+                                        const double (&sc)[D], const int (&w0)[D][D],
+                                        const int (&I)[D], const CellMap<D>& cm, const RKf& P,
                                         const float (&u0)[D], float (&out)[D]) {
     float v0[D];
 #pragma unroll
-    for (int cc = 0; cc < D; ++cc) v0[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, u0);
+    for (int cc = 0; cc < D; ++cc) v0[cc] = interp_tile<D, TS>(sV[cc], V, cc, sc[cc], w0[cc], I, cm, u0);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = u0[d] + v0[d];
@@ -194,7 +305,7 @@ __device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVe
 #pragma unroll
         for (int d = 0; d < D; ++d) u1[d] = fmaf(P.alpha, v0[d], u0[d]);
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) v1[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, u1);
+        for (int cc = 0; cc < D; ++cc) v1[cc] = interp_tile<D, TS>(sV[cc], V, cc, sc[cc], w0[cc], I, cm, u1);
         if (P.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = u0[d] + v1[d];
@@ -207,15 +318,15 @@ __device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVe
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + 0.5f * v0[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k2[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, q);
+        for (int cc = 0; cc < D; ++cc) k2[cc] = interp_tile<D, TS>(sV[cc], V, cc, sc[cc], w0[cc], I, cm, q);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + 0.5f * k2[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k3[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, q);
+        for (int cc = 0; cc < D; ++cc) k3[cc] = interp_tile<D, TS>(sV[cc], V, cc, sc[cc], w0[cc], I, cm, q);
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = u0[d] + k3[d];
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) k4[cc] = interp_tile<D, TS>(sV[cc], V, g, cc, sc[cc], t0, I, q);
+        for (int cc = 0; cc < D; ++cc) k4[cc] = interp_tile<D, TS>(sV[cc], V, cc, sc[cc], w0[cc], I, cm, q);
 #pragma unroll
         for (int d = 0; d < D; ++d)
             out[d] = u0[d] + (((v0[d] + 2.0f * k2[d]) + 2.0f * k3[d]) + k4[d]) / 6.0f;
@@ -225,10 +336,10 @@ __device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVe
 // advection of the particles in slots kb..ke-1 of tile `tile` of one grid (all
 // threads of the block call it; the LDS window sV is the block's)
 template <int D, int M, class TS>
-__device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, double dt,
-                                      const FPtr3& off, const uint8_t* __restrict__ index,
-                                      const RKf& P, int tile, int kb, int ke,
-                                      float (*sV)[TS::size()]) {
+__device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, const SMap<D>& m,
+                                      double dt, const FPtr3& off,
+                                      const uint8_t* __restrict__ index, const RKf& P, int tile,
+                                      int kb, int ke, float (*sV)[TS::size()]) {
     int t0[D];
     {
         int b = tile;
@@ -242,6 +353,11 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
     double sc[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) sc[c] = dt / g.h[c][0];
+    int w0[D][D];
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int d = 0; d < D; ++d) w0[c][d] = window_origin<D>(m, c, d, t0[d]);
     const int tid = threadIdx.x;
     // window fill: every load issued before the LDS stores
     constexpr int NI = (TS::size() + kBlock - 1) / kBlock;
@@ -257,7 +373,7 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
             bool ok = i < TS::size();
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const int k = t0[d] - 1 + r % TS::w(d);
+                const int k = w0[c][d] + r % TS::w(d);
                 r /= TS::w(d);
                 ok = ok && k >= 0 && k < V.n[c][d];
                 node += k * stride;
@@ -284,6 +400,8 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
         }
     }
     if (!in) return;
+    CellMap<D> cm;
+    cell_map<D>(m, I, cm);
     int64_t c = I[D - 1];
 #pragma unroll
     for (int d = D - 2; d >= 0; --d) c = c * g.ni[d] + I[d];
@@ -316,7 +434,7 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
                 for (int q = 1; q < B; ++q) v = j == q ? ub[q][d] : v;
                 u0[d] = v;
             }
-            advect_split_one<D, M, TS>(sV, V, g, sc, t0, I, P, u0, out);
+            advect_split_one<D, M, TS>(sV, V, sc, w0, I, cm, P, u0, out);
 #pragma unroll
             for (int q = 0; q < B; ++q)
 #pragma unroll
@@ -333,14 +451,24 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, dou
 // blockIdx.y: slot group of `span` slots (a multiple of 8), so that grids with
 // few tiles still fill the chip; each group stages its own velocity window
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, double dt,
+__global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V,
+                                                      const SplitGeom* __restrict__ geo, double dt,
                                                       FPtr3 off, const uint8_t* __restrict__ index,
                                                       RKf P, int span) {
     using TS = DefaultTile<D>;
     __shared__ float sV[D][TS::size()];
+    if (geo->bad) return;  // grid rejected: nothing changes
+    SMap<D> m;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m.r[c][d] = geo->r[c][d];
+            m.o[c][d] = geo->o[c][d];
+        }
     const int kb = blockIdx.y * span;
     const int ke = kb + span < g.K ? kb + span : g.K;
-    adv_split_tile<D, M, TS>(g, V, dt, off, index, P, blockIdx.x, kb, ke, sV);
+    adv_split_tile<D, M, TS>(g, V, m, dt, off, index, P, blockIdx.x, kb, ke, sV);
 }
 
 // ------------------------------------------------------------ ensemble
@@ -350,6 +478,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V, d
 struct EnsGrid {
     SGrid<2> g;
     SVelIn<2> V;
+    SMap<2> m;
     double dt;
     FPtr3 off;
     const uint8_t* index;
@@ -365,7 +494,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_split_ens(const EnsGrid* __restr
         const int i = tile_grid[t];
         const EnsGrid& E = grids[i];
         // zero_dt: the autotuner's timing runs (same work, positions unchanged)
-        adv_split_tile<2, M, TS>(E.g, E.V, zero_dt ? 0.0 : E.dt, E.off, E.index, P, t - tile0[i],
+        adv_split_tile<2, M, TS>(E.g, E.V, E.m, zero_dt ? 0.0 : E.dt, E.off, E.index, P, t - tile0[i],
                                  0, E.g.K,
                                  sV);
         __syncthreads();  // the window is refilled for the next tile
@@ -471,7 +600,9 @@ __global__ void __launch_bounds__(kBlock) k_p2g_split_cells(Grid<D> gg, SGrid<D>
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_p2g_split_nodes(Grid<D> gg, SGrid<D> g,
                                                             const double* __restrict__ part,
+                                                            const SplitGeom* __restrict__ geo,
                                                             double* __restrict__ F) {
+    if (geo->bad) return;  // grid rejected: F unchanged
     int64_t nn[D];
     int64_t NN = 1;
 #pragma unroll
@@ -550,9 +681,11 @@ __global__ void __launch_bounds__(kBlock) k_g2p_split(Grid<D> g, double* __restr
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_inj_scan_split(Grid<D> g, FPtr3 off,
                                                            const uint8_t* __restrict__ index,
-                                                           int min_q, InjectLists L) {
+                                                           int min_q, InjectLists L,
+                                                           const SplitGeom* __restrict__ geo) {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool valid = c < g.C;  // all lanes stay for the wavefront append
+    // all lanes stay for the wavefront append; a rejected grid lists no cell
+    const bool valid = c < g.C && !geo->bad;
     int64_t I[D];
     unlin<D>(g.ni, valid ? c : 0, I);
     int num = 0;
@@ -940,6 +1073,89 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> 
 }
 
 // ------------------------------------------------------------ host side
+}  // namespace
+
+namespace jp {
+
+int split_geometry(jp_ctx* ctx, const jp_cellgrid* gc, const jp_velgrid* vg) {
+    if (vg)
+        for (int c = 0; c < gc->ndim; ++c)
+            for (int d = 0; d < gc->ndim; ++d)
+                if (!vg->ax[c][d].x || vg->ax[c][d].n < 2 || vg->ax[c][d].n > (1 << 30))
+                    return fail(ctx, JP_ERR_ARG, "velocity axis%s %lld needs at least two points",
+                                "", c * 3 + d);
+    if (gc->ndim == 2) {
+        Grid<2> G = to_grid<2>(gc);
+        Vel<2> v{};
+        if (vg)
+            for (int c = 0; c < 2; ++c)
+                for (int d = 0; d < 2; ++d)
+                    v.ax[c][d] = Axis{vg->ax[c][d].x, vg->ax[c][d].dx, vg->ax[c][d].inv_dx,
+                                      vg->ax[c][d].n};
+        k_split_geom<2><<<1, kBlock, 0, ctx->stream>>>(G, v, vg != nullptr, dgeom(ctx));
+    } else {
+        Grid<3> G = to_grid<3>(gc);
+        Vel<3> v{};
+        if (vg)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d)
+                    v.ax[c][d] = Axis{vg->ax[c][d].x, vg->ax[c][d].dx, vg->ax[c][d].inv_dx,
+                                      vg->ax[c][d].n};
+        k_split_geom<3><<<1, kBlock, 0, ctx->stream>>>(G, v, vg != nullptr, dgeom(ctx));
+    }
+    JP_LAUNCHED(ctx);
+    return JP_OK;
+}
+
+namespace {
+int geom_reject(jp_ctx* ctx, int bits) {
+    // the rejection is reported now: clear the sticky word
+    int rc = zero_bytes(ctx, &dgeom(ctx)->sticky, sizeof(int32_t));
+    if (rc) return rc;
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const char* what = (bits & kGeomVertex) && (bits & kGeomVelocity)
+                           ? "vertex and velocity"
+                           : ((bits & kGeomVertex) ? "vertex" : "velocity");
+    return fail(ctx, JP_ERR_ARG,
+                "the FP32-split form needs uniform %s axes (deviation from x[0] + k*h above "
+                "1e-9 of the spacing h); the call changed nothing",
+                what);
+}
+}  // namespace
+
+int geom_reject_now(jp_ctx* ctx, int bits) { return geom_reject(ctx, bits); }
+
+// checks this call's geometry verdict now, whatever the synchronisation mode
+int split_finish_sync(jp_ctx* ctx) {
+    if (ctx->capturing) return JP_OK;
+    int32_t bad = 0;
+    JP_HIP(ctx, hipMemcpyAsync(&bad, &dgeom(ctx)->bad, sizeof bad, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return bad ? geom_reject(ctx, bad) : JP_OK;
+}
+
+int split_finish(jp_ctx* ctx) {
+    if (!ctx->sync || ctx->capturing) return JP_OK;
+    int32_t bad = 0;
+    JP_HIP(ctx, hipMemcpyAsync(&bad, &dgeom(ctx)->bad, sizeof bad, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return bad ? geom_reject(ctx, bad) : JP_OK;
+}
+
+int geom_report(jp_ctx* ctx) {
+    int32_t st = 0;
+    JP_HIP(ctx, hipMemcpyAsync(&st, &dgeom(ctx)->sticky, sizeof st, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st ? geom_reject(ctx, st) : JP_OK;
+}
+
+}  // namespace jp
+
+namespace {
+
 int check_common(jp_ctx* ctx, const jp_cellgrid* g, const void* const* off, const void* index) {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
@@ -965,16 +1181,11 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     SVelIn<D> sv;
     for (int c = 0; c < D; ++c) {
         sv.v[c] = V[c];
-        for (int d = 0; d < D; ++d) {
-            sv.n[c][d] = (int)vg->ax[c][d].n;
-            // standard staggering: the vertex axis along c, centres + ghosts elsewhere
-            const int64_t want = d == c ? G.ni[d] + 1 : G.ni[d] + 2;
-            if (vg->ax[c][d].n != want)
-                return fail(ctx, JP_ERR_ARG,
-                            "split advection needs the standard staggered grids%s (component %lld)",
-                            "", c);
-        }
+        for (int d = 0; d < D; ++d) sv.n[c][d] = (int)vg->ax[c][d].n;
     }
+    int rc = split_geometry(ctx, gc, vg);
+    if (rc) return rc;
+    const SplitGeom* geo = dgeom(ctx);
     RKf P;
     P.alpha = (float)alpha;
     P.half = alpha == 0.5;
@@ -985,16 +1196,19 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     for (int d = 0; d < D; ++d)
         tiles *= (G.ni[d] + DefaultTile<D>::n(d) - 1) / DefaultTile<D>::n(d);
     FPtr3 fp = fptr3<D>(off);
-    const int span = slot_span(tiles, G.K);
+    const int span = slot_span(ctx, tiles, G.K);
     const dim3 grid((unsigned)tiles, (unsigned)((G.K + span - 1) / span));
     if (method == JP_EULER)
-        k_adv_split<D, JP_EULER><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P, span);
+        k_adv_split<D, JP_EULER><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp, index, P,
+                                                                   span);
     else if (method == JP_RK2)
-        k_adv_split<D, JP_RK2><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P, span);
+        k_adv_split<D, JP_RK2><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp, index, P,
+                                                                 span);
     else
-        k_adv_split<D, JP_RK4><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, dt, fp, index, P, span);
+        k_adv_split<D, JP_RK4><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp, index, P,
+                                                                 span);
     JP_LAUNCHED(ctx);
-    return finish(ctx);
+    return split_finish(ctx);
 }
 
 template <int D>
@@ -1007,12 +1221,14 @@ int run_p2g(jp_ctx* ctx, const jp_cellgrid* gc, double* F, const double* Fp, flo
     int rc;
     if ((rc = ensure_scratch(ctx, (size_t)G.C * 8 * 2 * (1 << D)))) return rc;
     double* part = static_cast<double*>(ctx->scratch);
+    if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
     k_p2g_split_cells<D><<<(unsigned)row_blocks<D>(G), kBlock, 0, ctx->stream>>>(
         G, sg, Fp, fptr3<D>(off), index, part);
     JP_LAUNCHED(ctx);
-    k_p2g_split_nodes<D><<<(unsigned)blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, part, F);
+    k_p2g_split_nodes<D><<<(unsigned)blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, part,
+                                                                              dgeom(ctx), F);
     JP_LAUNCHED(ctx);
-    return finish(ctx);
+    return split_finish(ctx);
 }
 
 template <int D>
@@ -1040,7 +1256,9 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     InjectLists L;
     if ((rc = inject_lists(ctx, G.C, NQ, cap, D == 2 ? (size_t)G.C * 4 * 2 * D : 0, L))) return rc;
     FPtr3 fp = fptr3<D>(off);
-    k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, L);
+    if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
+    k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, L,
+                                                                     dgeom(ctx));
     JP_LAUNCHED(ctx);
     if ((rc = join_segments(ctx, NQ, L))) return rc;
     ArgPtrs ap;
@@ -1049,7 +1267,7 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     for (int ph = 0; ph < NQ; ++ph) {
         if constexpr (D == 2) {
             if (nargs <= 1) {
-                k_inj_wave2_split<<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+                k_inj_wave2_split<<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
                     G, sg, fp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
                     L.list + (int64_t)ph * cap, L.total + ph, L.occ,
                     static_cast<float*>(L.box), st);
@@ -1057,13 +1275,13 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
                 continue;
             }
         }
-        k_inj_wave_split<D><<<(unsigned)inject_blocks(cap), kBlock, 0, ctx->stream>>>(
+        k_inj_wave_split<D><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
             G, sg, fp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
             L.total + ph, st);
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;
-    return finish(ctx);
+    return split_finish(ctx);
 }
 
 // ------------------------------------------------------------ ensemble host
@@ -1146,6 +1364,9 @@ int jp_split_from_f64(jp_ctx* ctx, const jp_cellgrid* g, const double* const* co
     int rc = check_common(ctx, g, (const void* const*)off, index);
     if (rc) return rc;
     if (!coords) return fail(ctx, JP_ERR_ARG, "null coords%s");
+    // the split form is defined on uniform vertex axes only: reject others here
+    if ((rc = split_geometry(ctx, g, nullptr))) return rc;
+    if ((rc = split_finish_sync(ctx))) return rc;
     if (g->ndim == 2) {
         Grid<2> G = to_grid<2>(g);
         CPtr3 x{{coords[0], coords[1], nullptr}};
@@ -1256,16 +1477,23 @@ int jp_ensemble_create(jp_ctx* ctx, int n, const jp_cellgrid* grids, const jp_ve
         E.g = sgrid<2>(G);
         for (int c = 0; c < 2; ++c) {
             E.V.v[c] = V[2 * i + c];
-            for (int d = 0; d < 2; ++d) {
-                E.V.n[c][d] = (int)vgs[i].ax[c][d].n;
-                const int64_t want = d == c ? G.ni[d] + 1 : G.ni[d] + 2;
-                if (vgs[i].ax[c][d].n != want)
-                    return fail(ctx, JP_ERR_ARG, "ensemble grid%s %lld: non-standard staggering", "", i);
-            }
+            for (int d = 0; d < 2; ++d) E.V.n[c][d] = (int)vgs[i].ax[c][d].n;
         }
         E.dt = dt[i];
         E.off = FPtr3{{o2[0], o2[1], nullptr}};
         E.index = index[i];
+        // axis maps of this grid, checked now (the ensemble keeps its arrays)
+        if ((rc = split_geometry(ctx, g, vgs + i))) return rc;
+        SplitGeom geo;
+        JP_HIP(ctx, hipMemcpyAsync(&geo, dgeom(ctx), sizeof geo, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (geo.bad) return geom_reject_now(ctx, geo.bad);
+        for (int c = 0; c < 2; ++c)
+            for (int d = 0; d < 2; ++d) {
+                E.m.r[c][d] = geo.r[c][d];
+                E.m.o[c][d] = geo.o[c][d];
+            }
     }
     jp_ensemble* e = new jp_ensemble();
     e->n = n;

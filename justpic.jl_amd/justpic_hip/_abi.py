@@ -18,6 +18,7 @@ JP_EULER, JP_RK2, JP_RK4 = 0, 1, 2
 JP_MAX_ARGS = 16
 JP_ENS_VARIANTS = 8
 JP_ADJ_MAX_EVERY = 16
+JP_OPT_SLOT_SPAN, JP_OPT_INJECT_WAVES = 1, 2
 
 LIB_NAME = "libjustpic_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -81,6 +82,7 @@ _SIGS = {
     "jp_ctx_set_sync": (C.c_int, [_P, C.c_int]),
     "jp_ctx_set_stream": (C.c_int, [_P, _P]),
     "jp_ctx_stream": (_P, [_P]),
+    "jp_ctx_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
     "jp_sync": (C.c_int, [_P]),
     "jp_malloc": (C.c_int, [_P, C.POINTER(C.c_void_p), C.c_size_t]),
     "jp_free": (C.c_int, [_P, _P]),

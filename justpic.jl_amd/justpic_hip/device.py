@@ -48,6 +48,12 @@ class Context:
     def sync(self):
         self.check(self.lib.jp_sync(self.h), "sync")
 
+    def set_option(self, option: int, value: int):
+        """Launch option (``_abi.JP_OPT_*``; 0 restores the size-based choice).
+        Results do not depend on it: it lets small problems run the launch
+        shapes that large ones select."""
+        self.check(self.lib.jp_ctx_set_option(self.h, int(option), int(value)), "set_option")
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.jp_ctx_destroy(self.h)

@@ -49,6 +49,71 @@ def test_rk2_second_stage_weights_closed_form():
         assert abs(out - float(exact)) <= 4 * abs(np.spacing(float(exact)))
 
 
+def _fma(a, b, c):
+    """correctly rounded a*b + c (exact rationals, then one rounding)"""
+    from fractions import Fraction as Fr
+    return float(Fr(a) * Fr(b) + Fr(c))
+
+
+def _mul(a, b):
+    from fractions import Fraction as Fr
+    return float(Fr(a) * Fr(b))
+
+
+def test_rk2_blended_fold_follows_muladdmacro():
+    """Pins the fold of the blended second stage (src/Advection/RK2.jl:32-35),
+    p + s*dt*((1 - half*inv(a))*v0 + half*inv(a)*v1) under @muladd.
+
+    MuladdMacro (compat 0.2.4) folds a sum of products from the left: the
+    first product stays a plain product and each later one is fused into the
+    running sum; a product of more than two factors splits into (all but the
+    last factor, the last factor); summands without a product are added
+    first.  Its README documents exactly this:
+      a031*k1 + a032*k2            -> muladd(a032, k2, a031*k1)
+      bhat1*k1 + bhat4*k4 + ...    -> muladd(bhat10, k10, ... muladd(bhat4, k4, bhat1*k1))
+      uprev + dt*(...)             -> muladd(dt, (...), uprev)
+      update - dt*(...)            -> muladd(-dt, (...), update)
+    So the stage is muladd(s*dt, muladd(half*inv(a), v1, (1 - half*inv(a))*v0), p)
+    with 1 - half*inv(a) = muladd(-half, inv(a), 1) (the same value: half*inv(a)
+    is exact).  On inputs where fusing the first product instead rounds
+    differently, the oracle must give the documented value and not the other."""
+    rng = np.random.default_rng(2024)
+    differ = 0
+    trials = 400
+    for _ in range(trials):
+        a = float(rng.uniform(0.05, 0.95))
+        dt = float(rng.uniform(1e-3, 1.0))
+        v0, v1, p = (float(x) for x in rng.uniform(-300, 300, 3))
+        ia = 1.0 / a
+        w0 = _fma(-0.5, ia, 1.0)
+        w1 = _mul(0.5, ia)
+        doc = _fma(dt, _fma(w1, v1, _mul(w0, v0)), p)        # MuladdMacro's fold
+        other = _fma(dt, _fma(w0, v0, _mul(w1, v1)), p)      # first product fused
+        got = O.second_stage(a, dt, (v0,), (v1,), (p,))[0]
+        assert got == doc, (a, dt, v0, v1, p)
+        differ += got != other
+    # the two orders really differ on these inputs, so the check discriminates
+    assert differ > trials // 20, differ
+
+
+def test_first_stage_and_euler_folds():
+    """p + s*a*dt*v -> muladd(s*a*dt, v, p) (RK2.jl:13) and p + s*dt*v ->
+    muladd(s*dt, v, p) (Euler.jl:5), s = +1: one rounding of (a*dt) then one
+    fused multiply-add; the unfused p + fl(a*dt*v) differs on these inputs."""
+    rng = np.random.default_rng(7)
+    differ = 0
+    for _ in range(200):
+        a = float(rng.uniform(0.05, 0.95))
+        dt = float(rng.uniform(1e-3, 1.0))
+        v, p = (float(x) for x in rng.uniform(-300, 300, 2))
+        got = O.first_stage(O.JP_RK2, a, dt, (v,), (p,))[0]
+        assert got == _fma(_mul(a, dt), v, p)
+        differ += got != p + _mul(_mul(a, dt), v)
+        got_e = O.first_stage(O.JP_EULER, a, dt, (v,), (p,))[0]
+        assert got_e == _fma(dt, v, p)
+    assert differ > 10
+
+
 # test/test_interpolation_kernels.jl:47-60 --------------------------------
 def test_lerp_exact():
     assert O.lerp([1.0, 2.0], [0.5]) == 1.5
