@@ -62,11 +62,25 @@ def integrator(jp, spec):
     return jp.RungeKutta2() if spec["method"] == "rk2" else jp.RungeKutta4()
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cfg, spec, n_steps, seed):
-    """The oracle (sequential C, 1 core) on a bounded sample of the same
-    workload: n_steps full steps at the same size."""
+    """The oracle on all host threads (its OpenMP build: the cells of one
+    colour phase of move and inject, and the independent advection and
+    particle2grid loops, in parallel, as the reference's threaded CPU backend
+    runs them) on a bounded sample of the same workload: n_steps full steps at
+    the same size.  Returns (particle-steps/s, seconds, threads)."""
     from oracle import oracle as O
     from justpic_hip.configs import gaussian_T
+    O.set_threaded(True)
     xv, xi, V, dt = cfg["xv"], cfg["xi_vel"], cfg["V"], cfg["dt"]
     D = len(xv)
     nq = 2 ** D
@@ -87,15 +101,23 @@ def cpu_baseline(cfg, spec, n_steps, seed):
             O.inject_particles(hp, [pT], spec["min_xcell"], seed, it)
         O.particle2grid(hp, pT)
     el = time.perf_counter() - t0
-    return psteps / el, el
+    threads = O.threads()
+    O.set_threaded(False)
+    return psteps / el, el, threads
 
 
 CALLS = ("advection", "move", "inject", "p2g")
+# the kernels each call launches
+KERNELS = {"advection": "one advection kernel",
+           "move": "classify, 3^D colour phases, scatter, serial fallback",
+           "inject": "scan, list join, 2^D colour wavefront kernels",
+           "p2g": "per-cell partial sums and per-node gather (split) or occupancy words "
+                  "and per-node gather (FP64)"}
 
 
 def read_traffic(path):
-    """HBM bytes per launch of the advection kernel from a committed counter
-    summary (a line 'traffic_bytes_per_launch: N'), or None."""
+    """Measured HBM bytes per call (all of the call's launches) from a
+    committed counter summary (a line 'traffic_bytes_per_launch: N'), or None."""
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -232,6 +254,46 @@ def split_error(jp, ctx, cfg, args, spec):
     dis = int(len(ka) + len(kb) - 2 * common.size + np.count_nonzero(cella[ja] != cellb[jb]))
     return {"steps": args.check_steps, "particles": int(common.size), "max_abs_err": err,
             "domain": 1.0, "cell_disagreements": dis}
+
+
+def target_rows(jp, ctx, args):
+    """The stated target: the FP32-split RK2 step (advect + move + p2g) at 2D
+    256^2 (nxcell 24, max 48) and 3D 128^3 (nxcell 27, max 54), against the
+    FP64 drop-in on the same workload: throughput of both forms, their ratio,
+    the split step's algorithmic bytes against the HBM peak, and the split
+    position error after a few steps."""
+    from justpic_hip.configs import cellular3d
+    rows = {}
+    for D, n in ((2, 256), (3, 128)):
+        spec = dict(WORKLOADS[D])
+        spec["method"], spec["inject"] = "rk2", False
+        cfg = stream2d_cfg(n) if D == 2 else cellular3d(n, amp=100.0)
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.check_steps = min(args.steps, 20), 3, 10
+        log(f"target row {n}^{D} RK2")
+        r_s = run_form(jp, ctx, cfg, "split", sub, spec)
+        r_f = run_form(jp, ctx, cfg, "fp64", sub, spec)
+        err = split_error(jp, ctx, cfg, sub, spec)
+        rows[f"{D}d_{n}_rk2"] = {
+            "workload": f"{n}^{D} cells, nxcell {spec['nxcell']}, max_xcell {spec['max_xcell']}, "
+                        f"RK2 advection + move + p2g",
+            "particles": int(r_s["P"]),
+            "split": {"value": r_s["value"], "ms_per_step": r_s["ms_per_step"],
+                      "hbm_frac_step": r_s["hbm_frac_step"], "calls": r_s["calls"]},
+            "fp64": {"value": r_f["value"], "ms_per_step": r_f["ms_per_step"],
+                     "hbm_frac_step": r_f["hbm_frac_step"], "calls": r_f["calls"]},
+            "ratio_split_over_fp64": r_s["value"] / r_f["value"],
+            "split_error": err,
+            "target": {"hbm_frac": 0.6, "ratio": 1.7, "max_abs_err": 1e-6},
+        }
+        log("target", f"{n}^{D}", "ratio", rows[f"{D}d_{n}_rk2"]["ratio_split_over_fp64"],
+            "hbm", r_s["hbm_frac_step"], "err", err["max_abs_err"])
+    return rows
+
+
+def stream2d_cfg(n):
+    from justpic_hip.configs import stream2d
+    return stream2d(n, amp=250.0, readme=False)
 
 
 def run_ensemble(jp, args):
@@ -452,6 +514,8 @@ def main():
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--method", choices=("rk2", "rk4"), default=None,
                     help="integrator (default: RK2 for step, RK4 for step3d)")
+    ap.add_argument("--no-targets", action="store_true",
+                    help="skip the target rows (split vs FP64 RK2 at 256^2 and 128^3)")
     ap.add_argument("--cpu-steps", type=int, default=6,
                     help="oracle steps timed for the CPU baseline (0 = skip)")
     args = ap.parse_args()
@@ -503,31 +567,31 @@ def main():
         err = split_error(jp, ctx, cfg, args, spec)
         log("error", err)
 
-    # roofline: the advection kernel (one launch per call)
-    adv = main_r["calls"]["advection"]
+    # roofline: the dominant call of the step (the longest; its launches are
+    # bracketed by events on the context stream, which runs every kernel)
+    dom = max(main_r["calls"], key=lambda k: main_r["calls"][k]["us"])
+    adv = main_r["calls"][dom]
     tag = f"{args.n}" if D == 2 else f"{args.n}_3d"
     if spec["method"] != default_method:
         tag += f"_{spec['method']}"
     traffic = read_traffic(os.path.join(ROOT, "profiles",
-                                        f"traffic_advection_{args.form}_{tag}.txt"))
+                                        f"traffic_{dom}_{args.form}_{tag}.txt"))
     meth = spec["method"].upper()
     seq = f"advect {meth} + move" + (" + inject" if spec["inject"] else "") + " + p2g"
     size = f"{args.n}^{D}"
 
     cpu = None
-    # 3D: one step of the same workload at half the resolution (1/8 of the
-    # particles) keeps the sequential oracle near 10 s
+    # 3D: one step of the full-size workload
     cpu_steps = args.cpu_steps if D == 2 else min(args.cpu_steps, 1)
     if cpu_steps > 0:
         ccfg, csize = cfg, size
-        if D == 3:
-            from justpic_hip.configs import cellular3d
-            ccfg, csize = cellular3d(args.n // 2, amp=100.0), f"{args.n // 2}^3"
-        log("cpu baseline: oracle,", cpu_steps, "steps at", csize)
-        v_cpu, el = cpu_baseline(ccfg, spec, cpu_steps, args.seed)
-        cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{cpu_steps} full FP64 steps ({seq}) of the {csize} workload "
-                         f"by the sequential C oracle, {el:.1f} s"}
+        log("cpu baseline: threaded oracle,", cpu_steps, "steps at", csize)
+        v_cpu, el, nthr = cpu_baseline(ccfg, spec, cpu_steps, args.seed)
+        cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": nthr, "kind": "port",
+               "sample": f"{cpu_steps} full FP64 steps ({seq}) of the {csize} workload by "
+                         f"the C oracle on {nthr} OpenMP threads (cells of one colour "
+                         f"phase in parallel), {el:.1f} s, CPU {cpu_model()}; Julia is not "
+                         f"installed, so the reference itself cannot run"}
 
     names = {"split": "FP32-split positions", "fp64": "FP64 drop-in"}
     if D == 2:
@@ -555,7 +619,8 @@ def main():
         "hbm_frac_step": main_r["hbm_frac_step"],
         "step_bytes": main_r["bytes"]["step"],
         "calls": main_r["calls"],
-        "roofline": {"bound": "hbm", "kernel": f"advection {meth} ({args.form})",
+        "roofline": {"bound": "hbm", "kernel": f"{dom} call ({args.form}, "
+                                               f"{KERNELS[dom]})",
                      "achieved": adv["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": adv["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": adv["bytes"], "avg_launch_us": adv["us"]},
@@ -572,6 +637,8 @@ def main():
                                                     if args.form == "split"
                                                     else other["value"] / main_r["value"])}
         out["split_error"] = err
+    if args.case == "step" and not args.no_targets:
+        out["targets"] = target_rows(jp, ctx, args)
     print(json.dumps(out), flush=True)
 
 

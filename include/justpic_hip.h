@@ -115,6 +115,18 @@ int jp_fill_u8(jp_ctx* ctx, uint8_t* dptr, uint8_t value, int64_t count);
 int jp_fill_f64(jp_ctx* ctx, double* dptr, double value, int64_t count);
 int jp_fill_f32(jp_ctx* ctx, float* dptr, float value, int64_t count);
 
+/* CellArray layout conversion for host copies and checkpoints.  The device
+ * layout (B = 0, element (c, k) at data[c + C*k]) is the CPU CellArray data
+ * (1, K, C) (element (k, c) at data[k + K*c]) permuted by (3, 2, 1): what
+ * Array(particles) does before a JLD2 checkpoint (src/CellArrays/
+ * conversion.jl:32-43, src/IO/JLD2.jl:98-127) and what ROCArray(particles)
+ * undoes (ext/JustPICAMDGPUExt.jl:53-75).  Device to device, elements of 1,
+ * 4 or 8 bytes, src != dst. */
+int jp_cellarray_to_cpu_layout(jp_ctx* ctx, const void* src, void* dst, int64_t ncells,
+                               int K, int elem_bytes);
+int jp_cellarray_from_cpu_layout(jp_ctx* ctx, const void* src, void* dst, int64_t ncells,
+                                 int K, int elem_bytes);
+
 /* timing on the context stream (hipEvents) */
 int jp_event_create(jp_ctx* ctx, void** ev);
 int jp_event_destroy(jp_ctx* ctx, void* ev);
@@ -148,6 +160,44 @@ int jp_advection(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
                  const uint8_t* index, const double* const* V,
                  const jp_velgrid* vg, int method, double alpha, double dt);
 
+/* velocity interpolation schemes */
+#define JP_INTERP_LINEAR 0 /* interp_velocity2particle (advection.jl:93-148) */
+#define JP_INTERP_MQS 1    /* MQS (src/Interpolations/MQS.jl:2-158) */
+#define JP_INTERP_LINP 2   /* LinP (src/Particles/Advection/advection_LinP.jl:104-391) */
+
+/* advection_MQS!(particles, method, V, grid_vi, dt, dxi) and
+ * advection_LinP!(...) (src/Particles/Advection/advection_MQS.jl:30-100,
+ * advection_LinP.jl:27-88): jp_advection with the scheme's velocity
+ * interpolation, which applies where the bracket is interior along every
+ * axis of the component's grid (1 < idx < n - 1, 1-based) and falls back to
+ * the linear value elsewhere.  JP_INTERP_LINEAR is jp_advection. */
+int jp_advection_scheme(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
+                        const uint8_t* index, const double* const* V,
+                        const jp_velgrid* vg, int method, double alpha, double dt,
+                        int scheme);
+
+/* semilagrangian_advection!(F, F0, method, V, grid_vi, grid, dt) and its MQS
+ * and LinP forms (src/Particles/Advection/backtracking.jl:15-101,
+ * backtracking_MQS.jl:1-103, backtracking_LinP.jl:1-102).  grid: ndim nodal
+ * axes; F[f], F0[f]: nodal fields of dims grid[d].n, F[f] distinct from every
+ * F0.  Every interior node is traced back with RK2 or RK4 (velocity brackets
+ * seeded at the node index, no limit check) and F[f] there becomes the
+ * n-linear interpolation of F0[f] at the backtracked point; boundary nodes
+ * are not written.  The axes' dx arrays hold compute_dx
+ * (src/Particles/utils.jl:46-49: x[3] - x[2] for a range, diff(x) for a
+ * vector) and inv_dx = 1/dx.  di_at_node = 1 takes the normalisation spacing
+ * at the node instead of the backtracked cell (the linear NTuple method,
+ * backtracking.jl:79).  The reference's linear and LinP scalar methods read
+ * F while writing it (backtracking.jl:68, backtracking_LinP.jl:68); every
+ * method here reads F0.  A backtracked point outside the velocity or vertex
+ * grid, where the reference's bisection does not terminate, leaves its node
+ * unchanged and makes the call (or, asynchronously, the next jp_sync /
+ * jp_memcpy_d2h) return JP_ERR_ARG. */
+int jp_semilagrangian(jp_ctx* ctx, int ndim, const jp_axis* grid, const jp_velgrid* vg,
+                      const double* const* V, int nfields, double* const* F,
+                      const double* const* F0, int method, double alpha, double dt,
+                      int scheme, int di_at_node);
+
 /* move_particles!(particles, grid, args, dxi)
  * (src/Particles/move_safe.jl:23-125).  stats may be NULL (no host sync). */
 int jp_move_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords,
@@ -172,6 +222,44 @@ int jp_grid2particle(jp_ctx* ctx, const jp_cellgrid* g, double* Fp,
                      const double* F, const double* const* coords,
                      const uint8_t* index);
 
+/* cell_index(x::NTuple, xv::NTuple) for n points (src/particles.jl:205-224):
+ * per dimension d, out[d][i] = abs(trunc((x[d][i] - x0) / dx)) + 1 with
+ * x0 = xv[d].x[0] and dx = |xv[d].x[1] - xv[d].x[0]|, the 1-based index the
+ * reference returns.  x[d], out[d]: device arrays of n values; ndim 1..3;
+ * only the first two points of each axis are read. */
+int jp_cell_index(jp_ctx* ctx, int ndim, const jp_axis* xv, int64_t n,
+                  const double* const* x, int64_t* const* out);
+
+/* grid2particle_flip!(Fp, xvi, F, F0, particles; α)
+ * (src/Interpolations/grid_to_particle.jl:125-207): for every active slot,
+ * F_pic and F0_pic interpolated from the nodal F and F0 at
+ * t = (p - xv[I]) * inv(h), h = abs(minimum(diff(xv))) per dimension
+ * (grid_size, src/Interpolations/utils.jl:61-63), then
+ * Fp = muladd(F_pic, α, (Fp + F_pic - F0_pic) * (1 - α)); α = 1 is PIC,
+ * α = 0 FLIP.  nfields > 1 is the NTuple method (each field alike). */
+int jp_grid2particle_flip(jp_ctx* ctx, const jp_cellgrid* g, int nfields, double* const* Fp,
+                          const double* const* F, const double* const* F0,
+                          const double* const* coords, const uint8_t* index, double alpha);
+
+/* particle2centroid!(F, Fp, xci, particles, di)
+ * (src/Interpolations/particle_to_grid_centroid.jl:12-128): cell-centred F
+ * (dims ni) from the particles of each cell weighted about its centre
+ * (xc: ndim centre axes, particles.xci, xc[d].n == ni[d]).  tuple = 0 is the
+ * scalar method (nfields == 1), tuple = 1 the NTuple method, which the
+ * reference weights differently in 2D (inverse squared distance instead of
+ * the bilinear weight). */
+int jp_particle2centroid(jp_ctx* ctx, const jp_cellgrid* g, const jp_axis* xc, int nfields,
+                         double* const* F, const double* const* Fp,
+                         const double* const* coords, int tuple);
+
+/* centroid2particle!(Fp, xci, F, particles, di)
+ * (src/Interpolations/centroid_to_particle.jl:15-76): every slot whose
+ * coordinates are not NaN interpolates the cell-centred F (dims ni) on the
+ * centre grid at the cell shifted towards the particle (clamped inside). */
+int jp_centroid2particle(jp_ctx* ctx, const jp_cellgrid* g, const jp_axis* xc, int nfields,
+                         double* const* Fp, const double* const* F,
+                         const double* const* coords);
+
 /* inject_particles!(particles, args, grid, di)
  * (src/Particles/injection.jl:21-131, index_min_distance :330-393,
  * new_particle :411-417).  rand() is replaced by Philox4x32-10 keyed by
@@ -181,6 +269,42 @@ int jp_inject_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords
                         int min_xcell, uint64_t seed, uint64_t call_counter,
                         jp_stats* stats);
 
+/* inject_particles_phase!(particles, phases, args, fields, grid, grid_center,
+ * di, di_center) (src/Particles/injection.jl:146-325, kernel :200-231, body
+ * :233-325).  Same colours, quadrants, free-slot order and Philox stream as
+ * jp_inject_particles, but every short quadrant is topped up (the reference
+ * moves on to the next quadrant instead of breaking), a new particle takes
+ * the phase (phases: one double per slot) of its nearest neighbour, and each
+ * args[a] is interpolated from fields[a] at the new position, clamped to the
+ * extrema of the corner values: centred[a] = 1 for a cell-centred field
+ * (dims ni, read on the centre grid xc at the shifted index clamped to
+ * 1..ni-1), 0 for a nodal field (dims ni+1, the cell's corners).  xc: ndim
+ * centre axes (particles.xci with di.center), xc[d].n == ni[d].  A new
+ * particle with no neighbour keeps its phase value and counts as an orphan
+ * (the reference reads out of bounds there). */
+int jp_inject_particles_phase(jp_ctx* ctx, const jp_cellgrid* g, const jp_axis* xc,
+                              double* const* coords, uint8_t* index, double* phases,
+                              int nargs, double* const* args, const double* const* fields,
+                              const int* centred, int min_xcell, uint64_t seed,
+                              uint64_t call_counter, jp_stats* stats);
+
+/* update_phase_ratios!(phase_ratios, particles, phases)
+ * (src/PhaseRatios/utils.jl:15-43; centers.jl:14-30, vertices.jl:14-107,
+ * midpoints.jl:1-206).  Each output is one PhaseRatios field in the device
+ * CellArray layout, location l and phase k (1-based phase value k+1) at
+ * out[l + L*k] with L its location count:
+ *   center: dims ni;  vertex: dims ni+1;
+ *   faces[d] (Vx, Vy[, Vz]): dims ni + e_d;
+ *   midpoints (3D only; xy, yz, xz): dims ni + (1,1,0), (0,1,1), (1,0,1).
+ * A NULL output (or faces/midpoints array, or element) is skipped.  Face and
+ * midpoint locations without particles get 0, centres and vertices NaN, as
+ * in the reference.  xc: centre axes as for jp_inject_particles_phase. */
+#define JP_MAX_PHASES 16
+int jp_update_phase_ratios(jp_ctx* ctx, const jp_cellgrid* g, const jp_axis* xc,
+                           const double* const* coords, const double* phases, int nphases,
+                           double* center, double* vertex, double* const* faces,
+                           double* const* midpoints);
+
 /* ---- FP32-split form ---------------------------------------------------
  * No reference counterpart: a build-only precision form of the same path.
  * A position is the slot's cell (implicit) plus a float32 offset u in cell
@@ -188,10 +312,13 @@ int jp_inject_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords
  * Velocity interpolation and RK stages run in float32 in cell units; FP64 is
  * used for the conversions below, for the offset renormalisation when a
  * particle changes cell, and for the particle2grid weight sums.  Requires
- * uniform vertex axes and the standard staggering (velocity component c on
- * the vertex axis along c, on the centre axis with one ghost point per side
- * elsewhere, README.md:42-44).  Cell assignment is integer: a particle stays
- * in its cell while 0 <= u < 1 in every dimension. */
+ * uniform vertex axes and uniform velocity axes (any origin and spacing: each
+ * velocity axis is mapped affinely from its end points, so
+ * the README's expand_range ghost axes, README.md:48-55, are read as they
+ * are); every split entry point checks this on the device (deviation from
+ * x[0] + k*h at most 1e-9 of h) and a rejected grid changes nothing and
+ * returns JP_ERR_ARG.  Cell assignment is integer: a particle stays in its
+ * cell while 0 <= u < 1 in every dimension. */
 int jp_split_from_f64(jp_ctx* ctx, const jp_cellgrid* g, const double* const* coords,
                       const uint8_t* index, float* const* off);
 int jp_split_to_f64(jp_ctx* ctx, const jp_cellgrid* g, float* const* off,

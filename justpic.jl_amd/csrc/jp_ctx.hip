@@ -92,6 +92,57 @@ __global__ void k_fill_f32(float* p, float v, int64_t n) {
         p[i] = v;
 }
 
+// CellArray layouts: the device (B = 0) data of C cells x K slots is element
+// (c, k) at g[c + C k]; the CPU CellArray data (1, K, C) is element (k, c) at
+// h[k + K c], so converting is a transpose (src/CellArrays/conversion.jl:32-43
+// permutes the data by (3, 2, 1)).  One workgroup per 64 cells: the slot planes
+// are read 64 cells at a time (coalesced), the block's K x 64 output is one
+// contiguous run of the CPU layout.
+template <typename T>
+__global__ void __launch_bounds__(256) k_to_cpu_layout(const T* __restrict__ g, T* __restrict__ h,
+                                                       int64_t C, int K) {
+    __shared__ T tile[64][65];
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t nc = C - c0 < 64 ? C - c0 : 64;
+    for (int k = w; k < K; k += 4)
+        if (lane < nc) tile[lane][k] = g[c0 + lane + C * (int64_t)k];
+    __syncthreads();
+    const int64_t total = nc * K;
+    for (int64_t i = threadIdx.x; i < total; i += 256) {
+        const int cl = (int)(i / K), k = (int)(i - (int64_t)cl * K);
+        h[K * c0 + i] = tile[cl][k];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_from_cpu_layout(const T* __restrict__ h,
+                                                         T* __restrict__ g, int64_t C, int K) {
+    __shared__ T tile[64][65];
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t nc = C - c0 < 64 ? C - c0 : 64;
+    const int64_t total = nc * K;
+    for (int64_t i = threadIdx.x; i < total; i += 256) {
+        const int cl = (int)(i / K), k = (int)(i - (int64_t)cl * K);
+        tile[cl][k] = h[K * c0 + i];
+    }
+    __syncthreads();
+    for (int k = w; k < K; k += 4)
+        if (lane < nc) g[c0 + lane + C * (int64_t)k] = tile[lane][k];
+}
+
+template <typename T>
+void launch_layout(bool to_cpu, const void* src, void* dst, int64_t C, int K, hipStream_t s) {
+    const unsigned blocks = (unsigned)((C + 63) / 64);
+    if (to_cpu)
+        k_to_cpu_layout<T><<<blocks, 256, 0, s>>>(static_cast<const T*>(src), static_cast<T*>(dst),
+                                                 C, K);
+    else
+        k_from_cpu_layout<T><<<blocks, 256, 0, s>>>(static_cast<const T*>(src),
+                                                   static_cast<T*>(dst), C, K);
+}
+
 int64_t fill_grid(int64_t n) {
     int64_t b = blocks_for(n);
     return b < 1 ? 1 : (b > 65536 ? 65536 : b);
@@ -284,10 +335,18 @@ int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value) {
     }
 }
 
+// errors that asynchronous calls leave in device words: a split grid
+// rejection, a semi-Lagrangian backtrack that left the grid
+static int async_report(jp_ctx* ctx) {
+    if (ctx->capturing) return JP_OK;
+    int rc = geom_report(ctx);
+    return rc ? rc : range_sticky(ctx);
+}
+
 int jp_sync(jp_ctx* ctx) {
     if (!ctx) return JP_ERR_STATE;
     JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return ctx->capturing ? JP_OK : geom_report(ctx);
+    return async_report(ctx);
 }
 
 int jp_malloc(jp_ctx* ctx, void** p, size_t bytes) {
@@ -317,7 +376,7 @@ int jp_memcpy_d2h(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return JP_OK;
     JP_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return ctx->capturing ? JP_OK : geom_report(ctx);
+    return async_report(ctx);
 }
 
 int jp_memcpy_d2d(jp_ctx* ctx, void* dst, const void* src, size_t bytes) {
@@ -349,6 +408,33 @@ int jp_fill_f32(jp_ctx* ctx, float* p, float v, int64_t n) {
     k_fill_f32<<<fill_grid(n), kBlock, 0, ctx->stream>>>(p, v, n);
     JP_LAUNCHED(ctx);
     return finish(ctx);
+}
+
+static int cellarray_layout(jp_ctx* ctx, bool to_cpu, const void* src, void* dst, int64_t C,
+                            int K, int elem_bytes) {
+    if (!ctx) return JP_ERR_STATE;
+    if (!src || !dst) return fail(ctx, JP_ERR_ARG, "null layout buffer%s");
+    if (C < 0 || K < 1 || K > 64) return fail(ctx, JP_ERR_ARG, "layout needs C >= 0 and 1 <= K <= 64%s");
+    if (src == dst) return fail(ctx, JP_ERR_ARG, "layout conversion cannot run in place%s");
+    if (C == 0) return JP_OK;
+    switch (elem_bytes) {
+    case 1: launch_layout<uint8_t>(to_cpu, src, dst, C, K, ctx->stream); break;
+    case 4: launch_layout<uint32_t>(to_cpu, src, dst, C, K, ctx->stream); break;
+    case 8: launch_layout<uint64_t>(to_cpu, src, dst, C, K, ctx->stream); break;
+    default: return fail(ctx, JP_ERR_ARG, "element size must be 1, 4 or 8 bytes%s");
+    }
+    JP_LAUNCHED(ctx);
+    return finish(ctx);
+}
+
+int jp_cellarray_to_cpu_layout(jp_ctx* ctx, const void* src, void* dst, int64_t ncells, int K,
+                               int elem_bytes) {
+    return cellarray_layout(ctx, true, src, dst, ncells, K, elem_bytes);
+}
+
+int jp_cellarray_from_cpu_layout(jp_ctx* ctx, const void* src, void* dst, int64_t ncells, int K,
+                                 int elem_bytes) {
+    return cellarray_layout(ctx, false, src, dst, ncells, K, elem_bytes);
 }
 
 int jp_event_create(jp_ctx* ctx, void** ev) {

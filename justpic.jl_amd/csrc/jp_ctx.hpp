@@ -55,6 +55,16 @@ inline SplitGeom* dgeom(jp_ctx* ctx) {
     return reinterpret_cast<SplitGeom*>(static_cast<char*>(ctx->dsmall) + 1024);
 }
 
+// semilagrangian_advection: [0] set by this call's kernel when a backtracked
+// point left the grid, [1] the same, sticky until reported
+inline int* drange(jp_ctx* ctx) {
+    return reinterpret_cast<int*>(static_cast<char*>(ctx->dsmall) + 3072);
+}
+// report (and clear) an out-of-grid backtrack as JP_ERR_ARG
+int range_report(jp_ctx* ctx);
+// report one left by asynchronous semilagrangian calls
+int range_sticky(jp_ctx* ctx);
+
 // FP32-split entry points: launch the geometry check of the vertex axes (and,
 // when vg is not null, the affine maps of the velocity axes) into dgeom(ctx).
 // The call's kernels skip all work when it fails.

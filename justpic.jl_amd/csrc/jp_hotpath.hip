@@ -210,8 +210,10 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Grid<D> g, double* __restrict__ 
 // particles (src/Particles/injection.jl:100), and injection into a cell never
 // changes the particles of other cells.  One coalesced pass over all cells
 // therefore lists the only cells that can inject, per colour; the colour
-// phases then run the reference loop on those cells only.
-template <int D>
+// phases then run the reference loop on those cells only.  ANYQ (the phase
+// variant, which moves on to the next quadrant instead of breaking,
+// src/Particles/injection.jl:278): a cell is listed when any quadrant is short.
+template <int D, bool ANYQ = false>
 __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
                                                         const uint8_t* __restrict__ index,
                                                         int min_q, InjectLists L) {
@@ -220,6 +222,7 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
     int64_t I[D];
     unlin<D>(g.ni, valid ? c : 0, I);
     int num = 0;
+    int cnt_q[1 << D] = {};  // ANYQ: particles strictly inside quadrants 1..2^D-1
     if (valid) {
         double lo[D], hq[D];
 #pragma unroll
@@ -263,10 +266,23 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
                     }
                 }
                 num += inside ? 1 : 0;
+                if constexpr (ANYQ) {
+                    // the other quadrants (vertex + dq*bit, the wave kernel's bounds)
+#pragma unroll
+                    for (int q = 1; q < (1 << D); ++q) {
+                        bool in_q = on;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const double v = lo[d] + hq[d] * (double)((q >> d) & 1);
+                            in_q = in_q & (v < pb[j][d] && pb[j][d] < v + hq[d]);
+                        }
+                        cnt_q[q] += in_q ? 1 : 0;
+                    }
+                }
             }
         }
         L.occ[c] = act;
-        if (D == 2) {
+        if (D == 2 && L.box) {
             double* __restrict__ bx = static_cast<double*>(L.box);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -275,11 +291,15 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
             }
         }
     }
+    bool want = valid && num < min_q;
+    if constexpr (ANYQ) {
+#pragma unroll
+        for (int q = 1; q < (1 << D); ++q) want = want || (valid && cnt_q[q] < min_q);
+    }
     int col = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) col = col * 2 + (int)(I[d] & 1);
-    wave_segment<D == 2 ? 4 : 8>(valid && num < min_q, col, (int32_t)c, c >> 6, L.nwave, L.seg,
-                                 L.cnt);
+    wave_segment<D == 2 ? 4 : 8>(want, col, (int32_t)c, c >> 6, L.nwave, L.seg, L.cnt);
 }
 
 // _inject_particles! (src/Particles/injection.jl:68-131) with one wavefront per
@@ -289,13 +309,67 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
 // 3^D x K candidates lane-parallel, keeping the reference's first-occurrence
 // tie rule (strict '<' in loop order: neighbour cell, then slot) through a
 // (distance, order) argmin.
+//
+// PHASE: _inject_particles_phase! (src/Particles/injection.jl:233-325): every
+// short quadrant is topped up (`continue` past a full one, :278), the new
+// particle takes its nearest neighbour's phase, and each arg is interpolated
+// from its field (cell-centred or nodal) and clamped to the corner extrema.
+struct PhaseInj {
+    double* phases;
+    const double* fields[JP_MAX_ARGS];
+    uint32_t centred;  // bit a: fields[a] is cell-centred (dims ni), else nodal (ni + 1)
+    Axis xc[3];        // centre axes (particles.xci with di.center)
+};
+
+// Julia's min / max on Float64 (NaN-propagating, -0.0 below 0.0) and clamp
+__device__ inline double jl_min(double x, double y) {
+    if (x != x || y != y) return x != x ? x : y;
+    return (x < y || (x == y && signbit(x) && !signbit(y))) ? x : y;
+}
+__device__ inline double jl_max(double x, double y) {
+    if (x != x || y != y) return x != x ? x : y;
+    return (x > y || (x == y && !signbit(x) && signbit(y))) ? x : y;
+}
+
+// _grid2particle at corner cell idx of F (dims nn), t = (p - x0) * inv(di),
+// clamped to the extrema of the 2^D corner values (src/Particles/
+// injection.jl:299-314, cell_field :395-409)
 template <int D>
+__device__ double g2p_clamped(const double* __restrict__ F, const int64_t (&nn)[D],
+                              const int64_t (&idx)[D], const double (&p)[D],
+                              const double (&x0)[D], const double (&inv)[D]) {
+    double v[1 << D], t[D];
+#pragma unroll
+    for (int q = 0; q < (1 << D); ++q) {
+        int64_t o = 0, stride = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            o += (idx[d] + ((q >> d) & 1)) * stride;
+            stride *= nn[d];
+        }
+        v[q] = F[o];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) t[d] = (p[d] - x0[d]) * inv[d];
+    double r = lerp1(t[1], lerp1(t[0], v[0], v[1]), lerp1(t[0], v[2], v[3]));
+    if constexpr (D == 3)
+        r = lerp1(t[2], r, lerp1(t[1], lerp1(t[0], v[4], v[5]), lerp1(t[0], v[6], v[7])));
+    double lo = v[0], hi = v[0];
+#pragma unroll
+    for (int q = 1; q < (1 << D); ++q) {
+        lo = jl_min(lo, v[q]);
+        hi = jl_max(hi, v[q]);
+    }
+    return r > hi ? hi : (r < lo ? lo : r);
+}
+
+template <int D, bool PHASE = false>
 __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, uint8_t* index,
                                                         int nargs, ArgPtrs args, int min_q,
                                                         uint64_t seed, uint64_t counter,
                                                         const int32_t* __restrict__ list,
                                                         const int32_t* __restrict__ count,
-                                                        jp_stats* st) {
+                                                        jp_stats* st, PhaseInj pi) {
     const int lane = threadIdx.x & 63;
     // persistent wavefronts over the colour's list (every wavefront exits once
     // the list is exhausted)
@@ -334,7 +408,10 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
 #pragma unroll
         for (int d = 0; d < D; ++d) inside = inside & (vert[d] < p[d] && p[d] < vert[d] + dq[d]);
         int num = __popcll(__ballot(inside));
-        if (num >= min_q) break;
+        if (num >= min_q) {
+            if constexpr (PHASE) continue;
+            break;
+        }
         uint64_t freeS = ~__ballot(act) & full;
         bool done = false;
         while (freeS) {
@@ -424,9 +501,8 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
                     border = oo;
                 }
             }
-            if (border == 0x7fffffff) {
-                ++n_orph;
-            } else if (lane == ip) {
+            if (border == 0x7fffffff) ++n_orph;
+            if (lane == ip && border != 0x7fffffff) {
                 int64_t J[D];
                 int r = border >> 6;
 #pragma unroll
@@ -435,7 +511,35 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
                     r /= 3;
                 }
                 const int64_t esrc = lin<D>(g.ni, J) + C * (border & 63);
-                for (int a = 0; a < nargs; ++a) args.p[a][enew] = args.p[a][esrc];
+                if constexpr (PHASE)
+                    pi.phases[enew] = pi.phases[esrc];
+                else
+                    for (int a = 0; a < nargs; ++a) args.p[a][enew] = args.p[a][esrc];
+            }
+            if constexpr (PHASE) {
+                if (lane == ip) {
+                    // args from fields: nodal at this cell's corners, cell-centred at
+                    // the shifted index clamped to 1..ni-1 (1-based)
+                    int64_t nnv[D], nnc[D], ic[D], Iv[D];
+                    double xc0[D], ivc[D], xv0[D], ivv[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        nnv[d] = g.ni[d] + 1;
+                        nnc[d] = g.ni[d];
+                        Iv[d] = I[d];
+                        xv0[d] = xv[d];
+                        ivv[d] = g.xv[d].idx[I[d]];
+                        int64_t k = pn[d] < xv[d] + dq[d] ? I[d] - 1 : I[d];
+                        k = k > g.ni[d] - 2 ? g.ni[d] - 2 : (k < 0 ? 0 : k);
+                        ic[d] = k;
+                        xc0[d] = pi.xc[d].x[k];
+                        ivc[d] = pi.xc[d].idx[k];
+                    }
+                    for (int a = 0; a < nargs; ++a)
+                        args.p[a][enew] = ((pi.centred >> a) & 1)
+                                              ? g2p_clamped<D>(pi.fields[a], nnc, ic, pn, xc0, ivc)
+                                              : g2p_clamped<D>(pi.fields[a], nnv, Iv, pn, xv0, ivv);
+                }
             }
             // later searches of this wavefront read what lane ip just wrote
             __threadfence_block();
@@ -771,16 +875,98 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
         }
         k_inject_wave<D><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
             G, cp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
-            L.total + ph, st);
+            L.total + ph, st, PhaseInj{});
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;
     return finish(ctx);
 }
 
+// inject_particles_phase! (src/Particles/injection.jl:146-325): the same
+// scan / colour structure, cells listed when any quadrant is short
+template <int D>
+int run_inject_phase(jp_ctx* ctx, const jp_cellgrid* gc, const jp_axis* xc,
+                     double* const* coords, uint8_t* index, double* phases, int nargs,
+                     double* const* args, const double* const* fields, uint32_t centred,
+                     int min_xcell, uint64_t seed, uint64_t counter, jp_stats* stats) {
+    Grid<D> G = to_grid<D>(gc);
+    int rc;
+    if ((rc = stats_begin(ctx, stats))) return rc;
+    constexpr int NQ = D == 2 ? 4 : 8;
+    const int min_q = (min_xcell + NQ - 1) / NQ;
+    int64_t cap = 1;
+    for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
+    InjectLists L;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, 0, L))) return rc;
+    k_inject_scan<D, true><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D),
+                                                                        index, min_q, L);
+    JP_LAUNCHED(ctx);
+    if ((rc = join_segments(ctx, NQ, L))) return rc;
+    PhaseInj pi{};
+    pi.phases = phases;
+    pi.centred = centred;
+    for (int a = 0; a < nargs; ++a) pi.fields[a] = fields[a];
+    for (int d = 0; d < D; ++d) pi.xc[d] = Axis{xc[d].x, xc[d].dx, xc[d].inv_dx, xc[d].n};
+    Ptr3 cp = ptr3(coords, D);
+    ArgPtrs ap = argptrs(args, nargs);
+    jp_stats* st = stats ? dstats(ctx) : nullptr;
+    for (int ph = 0; ph < NQ; ++ph) {
+        k_inject_wave<D, true><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
+            G, cp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
+            L.total + ph, st, pi);
+        JP_LAUNCHED(ctx);
+    }
+    if ((rc = stats_end(ctx, stats))) return rc;
+    return finish(ctx);
+}
+
+// cell_index (src/particles.jl:205-217) for n points per dimension:
+// abs(trunc((x - x0) / dx)) + 1 with dx = |xv[2] - xv[1]| and x0 = first(xv)
+// (subtracted only when nonzero: x - 0.0 would turn -0.0 into 0.0, which the
+// result does not see either way).  1-based, as the reference returns it.
+struct CellIndexArgs {
+    const double* x[3];
+    int64_t* out[3];
+    const double* xv[3];  // vertex axes (only their first two points are read)
+};
+
+__global__ void __launch_bounds__(kBlock) k_cell_index(CellIndexArgs a, int ndim, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < ndim; ++d) {
+        const double x0 = a.xv[d][0];
+        const double dx = fabs(a.xv[d][1] - x0);
+        const double x = a.x[d][i];
+        const double q = x0 == 0.0 ? x / dx : (x - x0) / dx;
+        // unsafe_trunc(Int, trunc(q)): finite in-grid quotients only
+        const int64_t t = (int64_t)trunc(q);
+        a.out[d][i] = (t < 0 ? -t : t) + 1;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int jp_cell_index(jp_ctx* ctx, int ndim, const jp_axis* xv, int64_t n, const double* const* x,
+                  int64_t* const* out) {
+    if (!ctx) return JP_ERR_STATE;
+    if (ndim < 1 || ndim > 3) return fail(ctx, JP_ERR_ARG, "ndim must be 1, 2 or 3%s");
+    if (n < 0) return fail(ctx, JP_ERR_ARG, "negative point count%s");
+    if (!xv || !x || !out) return fail(ctx, JP_ERR_ARG, "null argument%s");
+    CellIndexArgs a{};
+    for (int d = 0; d < ndim; ++d) {
+        if (!x[d] || !out[d] || !xv[d].x || xv[d].n < 2)
+            return fail(ctx, JP_ERR_ARG, "bad axis or array%s (dim %lld)", "", d);
+        a.x[d] = x[d];
+        a.out[d] = out[d];
+        a.xv[d] = xv[d].x;
+    }
+    if (n == 0) return JP_OK;
+    k_cell_index<<<(unsigned)blocks_for(n), kBlock, 0, ctx->stream>>>(a, ndim, n);
+    JP_LAUNCHED(ctx);
+    return finish(ctx);
+}
 
 int jp_init_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, uint8_t* index,
                       int np_quadrant, uint64_t seed) {
@@ -846,6 +1032,40 @@ int jp_inject_particles(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords
     return g->ndim == 2
                ? run_inject<2>(ctx, g, coords, index, nargs, args, min_xcell, seed, call_counter, stats)
                : run_inject<3>(ctx, g, coords, index, nargs, args, min_xcell, seed, call_counter, stats);
+}
+
+int jp_inject_particles_phase(jp_ctx* ctx, const jp_cellgrid* g, const jp_axis* xc,
+                              double* const* coords, uint8_t* index, double* phases, int nargs,
+                              double* const* args, const double* const* fields,
+                              const int* centred, int min_xcell, uint64_t seed,
+                              uint64_t call_counter, jp_stats* stats) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if ((rc = check_ptrs(ctx, g->ndim, (const void* const*)coords, "coords"))) return rc;
+    if (!index || !phases || !xc) return fail(ctx, JP_ERR_ARG, "null index, phases or xc%s");
+    if (nargs < 0 || nargs > JP_MAX_ARGS) return fail(ctx, JP_ERR_ARG, "nargs out of range%s");
+    if (nargs && (rc = check_ptrs(ctx, nargs, (const void* const*)args, "args"))) return rc;
+    if (nargs && (rc = check_ptrs(ctx, nargs, (const void* const*)fields, "fields"))) return rc;
+    if (nargs && !centred) return fail(ctx, JP_ERR_ARG, "null centred flags%s");
+    if (min_xcell < 0) return fail(ctx, JP_ERR_ARG, "min_xcell must be non-negative%s");
+    uint32_t cmask = 0;
+    bool any_centred = false;
+    for (int a = 0; a < nargs; ++a)
+        if (centred[a]) {
+            cmask |= 1u << a;
+            any_centred = true;
+        }
+    for (int d = 0; d < g->ndim; ++d) {
+        if (any_centred && g->ni[d] < 2)
+            return fail(ctx, JP_ERR_ARG, "a cell-centred field needs two cells per dimension%s");
+        if (!xc[d].x || !xc[d].inv_dx || xc[d].n != g->ni[d])
+            return fail(ctx, JP_ERR_ARG, "centre axis%s %lld must have ni points", "", d);
+    }
+    return g->ndim == 2
+               ? run_inject_phase<2>(ctx, g, xc, coords, index, phases, nargs, args, fields, cmask,
+                                     min_xcell, seed, call_counter, stats)
+               : run_inject_phase<3>(ctx, g, xc, coords, index, phases, nargs, args, fields, cmask,
+                                     min_xcell, seed, call_counter, stats);
 }
 
 }  // extern "C"

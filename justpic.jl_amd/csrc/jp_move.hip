@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
     if (c >= g.C) return;
     const int kb = (int)(lb % gridDim.y) * span;
     const int ke = kb + span < g.K ? kb + span : g.K;
-    const int64_t C = g.C, S = C * g.K;
+    const int64_t C = g.C;
     // the first chunk's origins go out with the cell's words: unguarded loads
     // (slot clamped to K - 1, values used for arrivals only), one round trip
     // fewer than loading them after the arrival bits are known

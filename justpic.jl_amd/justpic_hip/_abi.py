@@ -19,6 +19,8 @@ JP_MAX_ARGS = 16
 JP_ENS_VARIANTS = 8
 JP_ADJ_MAX_EVERY = 16
 JP_OPT_SLOT_SPAN, JP_OPT_INJECT_WAVES = 1, 2
+JP_MAX_PHASES = 16
+JP_INTERP_LINEAR, JP_INTERP_MQS, JP_INTERP_LINP = 0, 1, 2
 
 LIB_NAME = "libjustpic_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
@@ -92,6 +94,8 @@ _SIGS = {
     "jp_fill_u8": (C.c_int, [_P, _P, C.c_uint8, C.c_int64]),
     "jp_fill_f64": (C.c_int, [_P, _P, C.c_double, C.c_int64]),
     "jp_fill_f32": (C.c_int, [_P, _P, C.c_float, C.c_int64]),
+    "jp_cellarray_to_cpu_layout": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int, C.c_int]),
+    "jp_cellarray_from_cpu_layout": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int, C.c_int]),
     "jp_event_create": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "jp_event_destroy": (C.c_int, [_P, _P]),
     "jp_event_record": (C.c_int, [_P, _P]),
@@ -104,6 +108,11 @@ _SIGS = {
     "jp_init_particles": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, C.c_uint64]),
     "jp_advection": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P,
                                C.POINTER(jp_velgrid), C.c_int, C.c_double, C.c_double]),
+    "jp_advection_scheme": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P,
+                                      C.POINTER(jp_velgrid), C.c_int, C.c_double, C.c_double,
+                                      C.c_int]),
+    "jp_semilagrangian": (C.c_int, [_P, C.c_int, _P, C.POINTER(jp_velgrid), _P, C.c_int, _P, _P,
+                                    C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
     "jp_move_particles": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
                                     C.POINTER(jp_stats)]),
     "jp_particle2grid": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P]),
@@ -112,6 +121,17 @@ _SIGS = {
     "jp_inject_particles": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
                                       C.c_int, C.c_uint64, C.c_uint64, C.POINTER(jp_stats)]),
     "jp_philox4x32_10": (None, [_P, _P, _P]),
+    "jp_inject_particles_phase": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P, C.c_int,
+                                            _P, _P, _P, C.c_int, C.c_uint64, C.c_uint64,
+                                            C.POINTER(jp_stats)]),
+    "jp_update_phase_ratios": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, C.c_int, _P,
+                                         _P, _P, _P]),
+    "jp_cell_index": (C.c_int, [_P, C.c_int, C.POINTER(jp_axis), C.c_int64, _P, _P]),
+    "jp_grid2particle_flip": (C.c_int, [_P, C.POINTER(jp_cellgrid), C.c_int, _P, _P, _P, _P, _P,
+                                        C.c_double]),
+    "jp_particle2centroid": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, C.c_int, _P, _P, _P,
+                                       C.c_int]),
+    "jp_centroid2particle": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, C.c_int, _P, _P, _P]),
     "jp_split_from_f64": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P]),
     "jp_split_to_f64": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P]),
     "jp_advection_split": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P,

@@ -104,6 +104,20 @@ class Axis:
         return f"Axis(n={self.n}, kind={self.kind}, [{self.first}, {self.last}])"
 
 
+def compute_dx(ax: Axis) -> Axis:
+    """The axis with the spacing rule of ``compute_dx``
+    (src/Particles/utils.jl:46-49), which ``semilagrangian_advection!`` uses
+    for its vertex and velocity grids: ``x[3] - x[2]`` (1-based) for a range,
+    ``diff(x)`` for a vector."""
+    o = Axis(ax.x, ax.kind)
+    if ax.kind == "range":
+        if ax.n < 3:
+            raise ValueError("compute_dx of a range needs at least three points")
+        o.dx = np.full(ax.n - 1, ax.x[2] - ax.x[1], dtype=np.float64)
+        o.inv_dx = 1.0 / o.dx
+    return o
+
+
 def expand_range(ax: Axis, sigdigits: int | None = None, lin: bool = True) -> Axis:
     """One ghost point on each side of a center axis.
 

