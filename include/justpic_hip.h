@@ -210,6 +210,20 @@ int jp_particle2grid(jp_ctx* ctx, const jp_cellgrid* g, double* F,
                      const double* Fp, const double* const* coords,
                      const uint8_t* index);
 
+/* particle2grid!(F, Fp, particles) for one scalar field, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under hipGraph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and same result, bit for bit, as
+ * jp_particle2grid.  Its only scratch, one 8-byte occupancy word per cell,
+ * lives in a fixed device array of JP_NOALLOC_MAX_CELLS words per GPU (16 MiB),
+ * kept until the process ends and shared by every context on that GPU: calls
+ * that overlap on one GPU (two streams or two contexts at once) race.  Grids
+ * with more than JP_NOALLOC_MAX_CELLS cells return JP_ERR_ARG. */
+#define JP_NOALLOC_MAX_CELLS (1 << 21)
+int jp_particle2grid_noalloc(jp_ctx* ctx, const jp_cellgrid* g, double* F,
+                             const double* Fp, const double* const* coords,
+                             const uint8_t* index);
+
 /* particle2grid!(F::NTuple, Fp::NTuple, particles)
  * (src/Interpolations/particle_to_grid.jl:70-109 2D, :153-199 3D). */
 int jp_particle2grid_multi(jp_ctx* ctx, const jp_cellgrid* g, int nfields,

@@ -148,6 +148,12 @@ int64_t fill_grid(int64_t n) {
     return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
 
+// Scratch of the allocation-free entry point (jp_particle2grid_noalloc): one
+// occupancy word per cell, JP_NOALLOC_MAX_CELLS words (16 MiB) per GPU, held
+// until the process ends.  Host code never passes this name to a kernel: it
+// uses the device address hipGetSymbolAddress returns (jp_ctx::noalloc).
+static __device__ __align__(256) uint64_t scratch[JP_NOALLOC_MAX_CELLS];
+
 }  // namespace
 
 namespace jp {
@@ -273,6 +279,13 @@ int jp_ctx_create(jp_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->dsmall, 4096);
     if (e == hipSuccess) e = hipMemset(ctx->dsmall, 0, 4096);
+    // this device's copy of the fixed scratch array (device `device` is current)
+    if (e == hipSuccess) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        ctx->noalloc = static_cast<uint64_t*>(p);
+        if (e == hipSuccess && !p) e = hipErrorInvalidSymbol;
+    }
     if (e == hipSuccess) {
         int cu = 0;
         if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&

@@ -24,6 +24,9 @@ struct jp_ctx {
     // launch options (jp_ctx_set_option); 0 = chosen from the problem size
     int opt_span = 0;
     int64_t opt_inject_waves = 0;
+    // device address of this GPU's fixed scratch array (JP_NOALLOC_MAX_CELLS
+    // words), looked up once at context creation
+    uint64_t* noalloc = nullptr;
 };
 
 namespace jp {

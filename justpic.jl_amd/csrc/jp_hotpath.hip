@@ -807,17 +807,23 @@ int run_init(jp_ctx* ctx, const jp_cellgrid* g, double* const* coords, uint8_t* 
     return finish(ctx);
 }
 
+// occ_fixed: caller-provided occupancy-word scratch of at least C words (the
+// allocation-free entry point); null: the context's growable scratch
 template <int D>
 int run_p2g(jp_ctx* ctx, const jp_cellgrid* g, int nf, double* const* F, const double* const* Fp,
-            const double* const* coords, const uint8_t* index, int mode) {
+            const double* const* coords, const uint8_t* index, int mode,
+            uint64_t* occ_fixed = nullptr) {
     Grid<D> G = to_grid<D>(g);
     int64_t NN = 1;
     for (int d = 0; d < D; ++d) NN *= G.ni[d] + 1;
     ArgPtrs Fa = argptrs(F, nf);
     ArgPtrs Fpa = argptrs(const_cast<double* const*>(Fp), nf);
     int rc;
-    if ((rc = ensure_scratch(ctx, (size_t)G.C * 8))) return rc;
-    uint64_t* occ = static_cast<uint64_t*>(ctx->scratch);
+    uint64_t* occ = occ_fixed;
+    if (!occ) {
+        if ((rc = ensure_scratch(ctx, (size_t)G.C * 8))) return rc;
+        occ = static_cast<uint64_t*>(ctx->scratch);
+    }
     if ((rc = occ_words(ctx, index, G.C, G.K, occ))) return rc;
     if (nf == 1)
         k_p2g<D, 1><<<blocks_for(NN), kBlock, 0, ctx->stream>>>(G, nf, Fa, Fpa, cptr3(coords, D),
@@ -992,6 +998,27 @@ int jp_particle2grid(jp_ctx* ctx, const jp_cellgrid* g, double* F, const double*
     int mode = g->ndim == 2 ? 0 : 1;
     return g->ndim == 2 ? run_p2g<2>(ctx, g, 1, Fa, Fpa, coords, index, mode)
                         : run_p2g<3>(ctx, g, 1, Fa, Fpa, coords, index, mode);
+}
+
+int jp_particle2grid_noalloc(jp_ctx* ctx, const jp_cellgrid* g, double* F, const double* Fp,
+                             const double* const* coords, const uint8_t* index) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if ((rc = check_ptrs(ctx, g->ndim, (const void* const*)coords, "coords"))) return rc;
+    if (!index || !F || !Fp) return fail(ctx, JP_ERR_ARG, "null field or index%s");
+    int64_t C = 1;
+    for (int d = 0; d < g->ndim; ++d) C *= g->ni[d];
+    if (C > JP_NOALLOC_MAX_CELLS)
+        return fail(ctx, JP_ERR_ARG,
+                    "particle2grid without allocation supports at most JP_NOALLOC_MAX_CELLS%s "
+                    "= 2097152 cells (got %lld)",
+                    "", (long long)C);
+    if (!ctx->noalloc) return fail(ctx, JP_ERR_STATE, "fixed scratch address unknown%s");
+    double* Fa[1] = {F};
+    const double* Fpa[1] = {Fp};
+    const int mode = g->ndim == 2 ? 0 : 1;
+    return g->ndim == 2 ? run_p2g<2>(ctx, g, 1, Fa, Fpa, coords, index, mode, ctx->noalloc)
+                        : run_p2g<3>(ctx, g, 1, Fa, Fpa, coords, index, mode, ctx->noalloc);
 }
 
 int jp_particle2grid_multi(jp_ctx* ctx, const jp_cellgrid* g, int nfields, double* const* F,

@@ -19,6 +19,7 @@ JP_MAX_ARGS = 16
 JP_ENS_VARIANTS = 8
 JP_ADJ_MAX_EVERY = 16
 JP_OPT_SLOT_SPAN, JP_OPT_INJECT_WAVES = 1, 2
+JP_NOALLOC_MAX_CELLS = 1 << 21
 JP_MAX_PHASES = 16
 JP_INTERP_LINEAR, JP_INTERP_MQS, JP_INTERP_LINP = 0, 1, 2
 
@@ -116,6 +117,7 @@ _SIGS = {
     "jp_move_particles": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
                                     C.POINTER(jp_stats)]),
     "jp_particle2grid": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P]),
+    "jp_particle2grid_noalloc": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P]),
     "jp_particle2grid_multi": (C.c_int, [_P, C.POINTER(jp_cellgrid), C.c_int, _P, _P, _P, _P]),
     "jp_grid2particle": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, _P, _P]),
     "jp_inject_particles": (C.c_int, [_P, C.POINTER(jp_cellgrid), _P, _P, C.c_int, _P,
