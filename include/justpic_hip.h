@@ -99,10 +99,15 @@ void* jp_ctx_stream(jp_ctx* ctx);
  *                       multiple of 8 up to 64 (>= max_xcell: whole cells).
  *   JP_OPT_INJECT_WAVES persistent wavefronts per injection colour launch: 0
  *                       (default) is one per listed cell up to 4096.
+ *   JP_OPT_GEOMETRY_RECHECK  the FP32-split axis check: 0 (default) once per
+ *                       set of axes (pointers and lengths) until any jp_free,
+ *                       1 on every call (for axes whose coordinates the
+ *                       caller changes in place or frees outside jp_free).
  * No reference counterpart (the KernelAbstractions launch sizes of the
  * reference are chosen by its backend). */
 #define JP_OPT_SLOT_SPAN 1
 #define JP_OPT_INJECT_WAVES 2
+#define JP_OPT_GEOMETRY_RECHECK 3
 int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value);
 int jp_sync(jp_ctx* ctx);
 int jp_malloc(jp_ctx* ctx, void** dptr, size_t bytes);

@@ -1,6 +1,8 @@
 // jp_ctx.hip -- context, memory, events and graph capture of the C-ABI.
 #include "jp_ctx.hpp"
 
+#include <atomic>
+
 #include <cstring>
 
 using namespace jp;
@@ -148,6 +150,8 @@ int64_t fill_grid(int64_t n) {
     return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
 
+std::atomic<uint64_t> g_free_gen{1};
+
 // Scratch of the allocation-free entry point (jp_particle2grid_noalloc): one
 // occupancy word per cell, JP_NOALLOC_MAX_CELLS words (16 MiB) per GPU, held
 // until the process ends.  Host code never passes this name to a kernel: it
@@ -157,6 +161,9 @@ static __device__ __align__(256) uint64_t scratch[JP_NOALLOC_MAX_CELLS];
 }  // namespace
 
 namespace jp {
+
+uint64_t free_generation() { return g_free_gen.load(std::memory_order_relaxed); }
+
 
 int ensure_scratch(jp_ctx* ctx, size_t bytes) {
     if (bytes <= ctx->scratch_bytes) return JP_OK;
@@ -279,6 +286,8 @@ int jp_ctx_create(jp_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->dsmall, 4096);
     if (e == hipSuccess) e = hipMemset(ctx->dsmall, 0, 4096);
+    if (e == hipSuccess) e = hipMalloc(&ctx->geo_dev, kGeomSlots * sizeof(SplitGeom));
+    if (e == hipSuccess) e = hipMemset(ctx->geo_dev, 0, kGeomSlots * sizeof(SplitGeom));
     // this device's copy of the fixed scratch array (device `device` is current)
     if (e == hipSuccess) {
         void* p = nullptr;
@@ -307,6 +316,7 @@ int jp_ctx_destroy(jp_ctx* ctx) {
     if (ctx->own) (void)hipStreamSynchronize(ctx->own);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->dsmall) (void)hipFree(ctx->dsmall);
+    if (ctx->geo_dev) (void)hipFree(ctx->geo_dev);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return JP_OK;
@@ -343,6 +353,11 @@ int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value) {
             return fail(ctx, JP_ERR_ARG, "injection wavefronts must be in 0..2^20%s");
         ctx->opt_inject_waves = value;
         return JP_OK;
+    case JP_OPT_GEOMETRY_RECHECK:
+        if (value < 0 || value > 1)
+            return fail(ctx, JP_ERR_ARG, "geometry recheck must be 0 or 1%s");
+        ctx->opt_geom_recheck = (int)value;
+        return JP_OK;
     default:
         return fail(ctx, JP_ERR_ARG, "unknown option%s %lld", "", option);
     }
@@ -372,7 +387,11 @@ int jp_malloc(jp_ctx* ctx, void** p, size_t bytes) {
 
 int jp_free(jp_ctx* ctx, void* p) {
     if (!ctx) return JP_ERR_STATE;
-    if (p) JP_HIP(ctx, hipFree(p));
+    if (p) {
+        // memory that held an axis may come back with other coordinates
+        g_free_gen.fetch_add(1, std::memory_order_relaxed);
+        JP_HIP(ctx, hipFree(p));
+    }
     return JP_OK;
 }
 

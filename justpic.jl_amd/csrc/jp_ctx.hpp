@@ -5,8 +5,28 @@
 
 #include <cstdio>
 #include <string>
+#include <unordered_map>
 
 #include "jp_common.hpp"
+
+// FP32-split geometry verdicts this context has computed: the axes they are
+// for (pointers and lengths) and the state of the verdict
+struct GeomKey {
+    int32_t ndim;
+    int32_t with_vel;
+    int64_t ni[3];
+    jp_axis xv[3];
+    jp_axis vel[3][3];
+};
+struct GeomEntry {
+    GeomKey key;
+    int state = 0;          // 0 empty, 1 accepted (read back), 2 launched under capture
+    bool standard = false;  // r == 1 and o == (c == d ? 0 : 1/2) exactly
+    bool pinned = false;    // read by a captured graph: never reused for other axes
+    uint64_t gen = 0;       // jp_free generation when checked
+    uint64_t used = 0;      // last use (least recently used slot is reused)
+};
+constexpr int kGeomSlots = 256;
 
 struct jp_ctx {
     int device = 0;
@@ -24,6 +44,14 @@ struct jp_ctx {
     // launch options (jp_ctx_set_option); 0 = chosen from the problem size
     int opt_span = 0;
     int64_t opt_inject_waves = 0;
+    // split geometry: device verdicts (kGeomSlots SplitGeom), the host side
+    // of each, the slot of the current call, JP_OPT_GEOMETRY_RECHECK
+    void* geo_dev = nullptr;
+    GeomEntry geo_cache[kGeomSlots];
+    std::unordered_map<uint64_t, int> geo_index;  // hash of the key -> slot
+    int geo_cur = 0;
+    uint64_t geo_clock = 0;
+    int opt_geom_recheck = 0;
     // device address of this GPU's fixed scratch array (JP_NOALLOC_MAX_CELLS
     // words), looked up once at context creation
     uint64_t* noalloc = nullptr;
@@ -54,9 +82,21 @@ inline jp_stats* dstats(jp_ctx* ctx) { return reinterpret_cast<jp_stats*>(ctx->d
 inline int* dflags(jp_ctx* ctx) {
     return reinterpret_cast<int*>(static_cast<char*>(ctx->dsmall) + 256);
 }
+// the sticky rejection word of asynchronous split calls lives here
 inline SplitGeom* dgeom(jp_ctx* ctx) {
     return reinterpret_cast<SplitGeom*>(static_cast<char*>(ctx->dsmall) + 1024);
 }
+// the verdict (and affine maps) of the current split call's axes
+inline SplitGeom* geo_ptr(jp_ctx* ctx) {
+    return static_cast<SplitGeom*>(ctx->geo_dev) + ctx->geo_cur;
+}
+// the current call's axes were accepted and have the standard staggering
+inline bool geo_standard(const jp_ctx* ctx) {
+    const GeomEntry& e = ctx->geo_cache[ctx->geo_cur];
+    return e.state == 1 && e.standard;
+}
+// incremented by every jp_free (any context): verdicts of older axes are not reused
+uint64_t free_generation();
 
 // semilagrangian_advection: [0] set by this call's kernel when a backtracked
 // point left the grid, [1] the same, sticky until reported
@@ -68,9 +108,13 @@ int range_report(jp_ctx* ctx);
 // report one left by asynchronous semilagrangian calls
 int range_sticky(jp_ctx* ctx);
 
-// FP32-split entry points: launch the geometry check of the vertex axes (and,
-// when vg is not null, the affine maps of the velocity axes) into dgeom(ctx).
-// The call's kernels skip all work when it fails.
+// FP32-split entry points: the geometry check of the vertex axes (and, when vg
+// is not null, the affine maps of the velocity axes) into geo_ptr(ctx).  Axes
+// this context accepted before (same pointers and lengths, no jp_free since)
+// are not checked again.  Otherwise the check runs and its verdict is read
+// back: rejected axes return JP_ERR_ARG before any kernel of the call runs.
+// Under graph capture the verdict cannot be read: the call's kernels skip all
+// work when it fails, and the next jp_sync reports it.
 int split_geometry(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg);
 // finish() of a split entry point: a synchronous call reports a rejected
 // grid as JP_ERR_ARG; an asynchronous one leaves it in the sticky word, which

@@ -3,6 +3,7 @@
 // B = 0 cell-plane layout data[c + C*k] (advection: jp_advect.hip, move:
 // jp_move.hip).
 #include "jp_ctx.hpp"
+#include "jp_inject2.hpp"
 
 #include <cmath>
 
@@ -558,208 +559,6 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
     }
 }
 
-// The same injection for 2D with at most one field, loading only what the
-// nearest-neighbour search can use.  The scan wrote every cell's occupancy
-// word and the box of its particles; a wavefront loads its cell's particles
-// (active slots only), and a neighbour cell's particles only when the
-// distance from the new particle to that cell's box does not exceed the best
-// distance found so far (the box bound is computed with the same floating
-// point operations as the distances, so the pruned search returns exactly the
-// reference's first-occurrence minimum).  Loaded neighbours stay in
-// registers for the cell's later injections (neighbours do not change during
-// a colour phase), and copies come from registers or one uniform load.
-__global__ void __launch_bounds__(kBlock) k_inject_wave2(Grid<2> g, Ptr3 coords, uint8_t* index,
-                                                         int nargs, double* __restrict__ arg,
-                                                         int min_q, uint64_t seed,
-                                                         uint64_t counter,
-                                                         const int32_t* __restrict__ list,
-                                                         const int32_t* __restrict__ count,
-                                                         uint64_t* __restrict__ occw,
-                                                         double* __restrict__ box,
-                                                         jp_stats* st) {
-    constexpr int NB = 9, CENTER = 4;
-    const int lane = threadIdx.x & 63;
-    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
-    const int64_t total = *count;
-    const int64_t C = g.C;
-    const int K = g.K;
-    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
-    int64_t n_inj = 0, n_orph = 0;
-    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
-         w += nwv) {
-        const int64_t c = __builtin_amdgcn_readfirstlane(list[w]);
-        int64_t I[2];
-        unlin<2>(g.ni, c, I);
-        double xv[2], dq[2];
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            xv[d] = g.xv[d].x[I[d]];
-            dq[d] = g.xv[d].dx[I[d]] / 2.0;
-        }
-        int64_t cn[NB];
-        uint64_t o9[NB];
-        double bl[NB][2], bh[NB][2];
-        bool okn[NB];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
-            okn[nb] = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
-            cn[nb] = okn[nb] ? J0 + g.ni[0] * J1 : c;  // a valid cell either way
-        }
-        // unguarded loads, all issued before any is used; masked afterwards
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            o9[nb] = occw[cn[nb]];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                bl[nb][d] = box[(2 * d) * C + cn[nb]];
-                bh[nb][d] = box[(2 * d + 1) * C + cn[nb]];
-            }
-        }
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
-        const int64_t e = c + C * lane;
-        bool act = (o9[CENTER] >> lane) & 1;
-        double q[NB][2];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) q[nb][0] = q[nb][1] = 0.0;
-        q[CENTER][0] = act ? coords.p[0][e] : 0.0;
-        q[CENTER][1] = act ? coords.p[1][e] : 0.0;
-        double a_c = (act && nargs) ? arg[e] : 0.0;
-        // this cell's new particles stay in registers until the cell is done
-        // (a load issued after a store would also wait for the store)
-        bool inj = false, copied = false;
-        uint32_t loaded = 1u << CENTER;
-        for (int qd = 0; qd < 4; ++qd) {
-            double vert[2];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) vert[d] = xv[d] + dq[d] * (double)((qd >> d) & 1);
-            bool inside = act;
-#pragma unroll
-            for (int d = 0; d < 2; ++d)
-                inside = inside & (vert[d] < q[CENTER][d] && q[CENTER][d] < vert[d] + dq[d]);
-            int num = __popcll(__ballot(inside));
-            if (num >= min_q) break;
-            uint64_t freeS = ~__ballot(act) & full;
-            while (freeS) {
-                const int ip = __builtin_ctzll(freeS);
-                freeS &= freeS - 1;
-                ++num;
-                double pn[2];
-#pragma unroll
-                for (int d = 0; d < 2; ++d) {
-                    const double r = uniform53(seed, (uint64_t)c, ip, d, 2, counter);
-                    pn[d] = vert[d] + dq[d] * fma(0.95, r, 0.05);
-                }
-                bl[CENTER][0] = fmin(bl[CENTER][0], pn[0]);
-                bh[CENTER][0] = fmax(bh[CENTER][0], pn[0]);
-                bl[CENTER][1] = fmin(bl[CENTER][1], pn[1]);
-                bh[CENTER][1] = fmax(bh[CENTER][1], pn[1]);
-                if (lane == ip) {
-                    act = true;
-                    inj = true;
-                    q[CENTER][0] = pn[0];
-                    q[CENTER][1] = pn[1];
-                }
-                // index_min_distance over the loaded cells: first occurrence
-                // of the minimum in (cell, slot) order
-                double best;
-                int border;
-                // best holds the smallest SQUARED distance: the reference's
-                // argmin of sqrt(s) with strict < (first in order on ties) is
-                // the argmin of s, except that a larger s can round to the same
-                // sqrt and come first; such near ties are resolved exactly
-                auto search = [&]() __attribute__((always_inline)) {
-                    best = __builtin_inf();
-                    border = 0x7fffffff;
-                    double s9[NB];
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) {
-                        s9[nb] = __builtin_inf();
-                        if (!((loaded >> nb) & 1)) continue;
-                        const bool av = nb == CENTER ? (act && lane != ip) : ((o9[nb] >> lane) & 1);
-                        if (!av) continue;
-                        if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
-                        const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
-                        s9[nb] = du * du + dv * dv;
-                        if (s9[nb] < best) {
-                            best = s9[nb];
-                            border = nb * 64 + lane;
-                        }
-                    }
-                    wave_argmin(best, border);
-                    resolve_ties(s9, best, border, lane);
-                };
-                search();
-                // neighbour cells whose box is not farther than the best so far
-                uint32_t need = 0;
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    if (nb == CENTER || ((loaded >> nb) & 1) || !o9[nb]) continue;
-                    const double gx = bl[nb][0] > pn[0] ? bl[nb][0] - pn[0]
-                                                        : (pn[0] > bh[nb][0] ? pn[0] - bh[nb][0] : 0.0);
-                    const double gy = bl[nb][1] > pn[1] ? bl[nb][1] - pn[1]
-                                                        : (pn[1] > bh[nb][1] ? pn[1] - bh[nb][1] : 0.0);
-                    if (!(gx * gx + gy * gy > best * kPrune)) need |= 1u << nb;
-                }
-                if (need) {
-                    // one predicate per lane (no uniform branch per cell), so the
-                    // loads of all needed cells are in flight together
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) {
-                        if (nb == CENTER) continue;
-                        const bool take = ((need >> nb) & 1) && ((o9[nb] >> lane) & 1);
-                        const int64_t e2 = cn[nb] + C * lane;
-                        q[nb][0] = take ? coords.p[0][e2] : q[nb][0];
-                        q[nb][1] = take ? coords.p[1][e2] : q[nb][1];
-                    }
-                    loaded |= need;
-                    search();
-                }
-                if (border == 0x7fffffff) {
-                    ++n_orph;  // no neighbour at all: the field is left as it was
-                } else if (nargs) {
-                    const int nbw = border >> 6, lw = border & 63;  // uniform
-                    double v;
-                    if (nbw == CENTER) {
-                        v = __shfl(a_c, lw, 64);
-                    } else {
-                        int64_t cs = cn[0];
-#pragma unroll
-                        for (int nb = 1; nb < NB; ++nb) cs = nb == nbw ? cn[nb] : cs;
-                        v = arg[cs + C * lw];
-                    }
-                    if (lane == ip) {
-                        a_c = v;
-                        copied = true;
-                    }
-                }
-                ++n_inj;
-                if (num >= min_q) break;
-            }
-        }
-        if (inj) {
-            coords.p[0][e] = q[CENTER][0];
-            coords.p[1][e] = q[CENTER][1];
-            index[e] = 1;
-            if (copied) arg[e] = a_c;
-        }
-        // later colour phases see this cell's final occupancy and box
-        const uint64_t fin = __ballot(act) & full;
-        if (lane == 0) {
-            occw[c] = fin;
-            box[c] = bl[CENTER][0];
-            box[C + c] = bh[CENTER][0];
-            box[2 * C + c] = bl[CENTER][1];
-            box[3 * C + c] = bh[CENTER][1];
-        }
-    }  // work list
-    if (lane == 0) {
-        stat_add(st, ST_INJECTED, n_inj);
-        stat_add(st, ST_ORPHAN, n_orph);
-    }
-}
-
 // ============================================================ host side
 int check_ptrs(jp_ctx* ctx, int n, const void* const* p, const char* what) {
     if (!p) return fail(ctx, JP_ERR_ARG, "null %s array", what);
@@ -858,7 +657,7 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
     int64_t cap = 1;
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
-    if ((rc = inject_lists(ctx, G.C, NQ, cap, D == 2 ? (size_t)G.C * 8 * 2 * D : 0, L))) return rc;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, 0, L))) return rc;
     k_inject_scan<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D), index,
                                                                   min_q, L);
     JP_LAUNCHED(ctx);
@@ -871,10 +670,20 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
         // persistent wavefronts, one listed cell at a time each
         if constexpr (D == 2) {
             if (nargs <= 1) {
-                k_inject_wave2<<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
-                    G, cp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
-                    L.list + (int64_t)ph * cap, L.total + ph, L.occ,
-                    static_cast<double*>(L.box), st);
+                Inject2 a{};
+                a.g = G;
+                a.x[0] = coords[0];
+                a.x[1] = coords[1];
+                a.index = index;
+                a.arg = nargs ? args[0] : nullptr;
+                a.min_q = min_q;
+                a.seed = seed;
+                a.counter = counter;
+                a.list = L.list + (int64_t)ph * cap;
+                a.count = L.total + ph;
+                a.occw = L.occ;
+                a.st = st;
+                k_inject2<false><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(a);
                 JP_LAUNCHED(ctx);
                 continue;
             }

@@ -1,0 +1,243 @@
+// jp_inject2.hpp -- 2D injection with at most one field, FP64 and FP32-split
+// positions (inject_particles!, src/Particles/injection.jl:68-131,
+// index_min_distance :330-393).
+//
+// One wavefront per listed cell, lanes = slots (K <= 64).  The reference
+// injects one particle at a time and searches its nearest neighbour among
+// the particles present at that moment, which include the cell's earlier
+// injections, so a cell's searches form a chain.  Nothing in a search feeds
+// the next one's inputs, though: which slots are filled, in which quadrant,
+// follows from the quadrant counts alone; each new position is a function of
+// (cell, slot); and the candidates of injection j are the particles present
+// before the call plus the injections before j.  So the wavefront first plans
+// the whole cell with ballots (quadrants in order, the lowest free slots, the
+// count of each quadrant including what earlier quadrants added), draws every
+// new position in its own lane, loads the 3x3 neighbourhood once (positions
+// and field values), and then runs the searches back to back on registers:
+// per injection a lane-parallel distance pass, a wavefront argmin on squared
+// distances with exact tie resolution (first occurrence of the minimal
+// rounded distance in (neighbour cell, slot) order), and the field copy,
+// which reads the source lane's current value (an earlier injection's copy
+// when the nearest particle is one of them).
+#pragma once
+
+#include <type_traits>
+
+#include "jp_ctx.hpp"
+
+namespace jp {
+
+struct Inject2 {
+    Grid<2> g;
+    double* x[2];      // FP64 form: coordinates
+    float* u[2];       // split form: offsets in cell units
+    double h[2];       // split form: uniform vertex spacing
+    uint8_t* index;
+    double* arg;       // the field moved with the particles, or null
+    int min_q;         // particles wanted per quadrant
+    uint64_t seed, counter;
+    const int32_t* list;   // this colour's listed cells
+    const int32_t* count;  // their number
+    uint64_t* occw;        // occupancy word of every cell (updated here)
+    jp_stats* st;
+};
+
+// a uniform lane's double (v_readlane on both halves)
+__device__ inline double readlane_d(double v, int l) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline float readlane_f(float v, int l) {
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), l));
+}
+
+// the lowest n set bits of m
+__device__ inline uint64_t lowest_bits(uint64_t m, int n) {
+    uint64_t t = 0;
+    for (int k = 0; k < n && m; ++k) {
+        const uint64_t b = m & (~m + 1);
+        t |= b;
+        m ^= b;
+    }
+    return t;
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
+    using P = typename std::conditional<SPLIT, float, double>::type;
+    constexpr int NB = 9, CENTER = 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
+    const int64_t total = *a.count;
+    const Grid<2>& g = a.g;
+    const int64_t C = g.C;
+    const int K = g.K;
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    const bool slot = lane < K;
+    const bool has_arg = a.arg != nullptr;
+    int64_t n_inj = 0, n_orph = 0;
+    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
+         w += nwv) {
+        const int64_t c = __builtin_amdgcn_readfirstlane(a.list[w]);
+        int64_t I[2];
+        unlin<2>(g.ni, c, I);
+        int64_t cn[NB];
+        bool okn[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
+            okn[nb] = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
+            cn[nb] = okn[nb] ? J0 + g.ni[0] * J1 : c;  // a valid cell either way
+        }
+        uint64_t o9[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) o9[nb] = a.occw[cn[nb]];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
+        // the whole neighbourhood in one round of loads: positions of the
+        // active slots, field values (this cell's for every slot: an orphan
+        // injection keeps the value its slot holds)
+        P q[NB][2];
+        double f[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const bool take = (o9[nb] >> lane) & 1;
+            const int64_t e2 = cn[nb] + C * lane;
+            if constexpr (SPLIT) {
+                q[nb][0] = take ? a.u[0][e2] : 0.0f;
+                q[nb][1] = take ? a.u[1][e2] : 0.0f;
+            } else {
+                q[nb][0] = take ? a.x[0][e2] : 0.0;
+                q[nb][1] = take ? a.x[1][e2] : 0.0;
+            }
+            const bool takef = has_arg && (nb == CENTER ? slot : take);
+            f[nb] = takef ? a.arg[e2] : 0.0;
+        }
+        const bool act0 = (o9[CENTER] >> lane) & 1;
+        double xv[2], dq[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            xv[d] = g.xv[d].x[I[d]];
+            dq[d] = g.xv[d].dx[I[d]] / 2.0;
+        }
+        // plan: the quadrants in order, each topped up from the lowest free
+        // slots until it holds min_q particles; the first full quadrant ends
+        // the cell (injection.jl:87-126)
+        uint64_t injm = 0;
+        bool act = act0;
+        for (int qd = 0; qd < 4; ++qd) {
+            bool inside = act;
+            if constexpr (SPLIT) {
+                const float q0 = 0.5f * (float)(qd & 1), q1 = 0.5f * (float)((qd >> 1) & 1);
+                inside = inside && (q0 < q[CENTER][0] && q[CENTER][0] < q0 + 0.5f) &&
+                         (q1 < q[CENTER][1] && q[CENTER][1] < q1 + 0.5f);
+            } else {
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    const double v = xv[d] + dq[d] * (double)((qd >> d) & 1);
+                    inside = inside && (v < q[CENTER][d] && q[CENTER][d] < v + dq[d]);
+                }
+            }
+            const int num = __popcll(__ballot(inside));
+            if (num >= a.min_q) break;
+            const uint64_t t = lowest_bits(~__ballot(act) & full, a.min_q - num);
+            injm |= t;
+            if ((t >> lane) & 1) {
+                // new_particle: vertex + dq * muladd(0.95, rand, 0.05)
+                act = true;
+                if constexpr (SPLIT) {
+                    const double q0 = 0.5 * (double)(qd & 1), q1 = 0.5 * (double)((qd >> 1) & 1);
+                    q[CENTER][0] = (float)(q0 + 0.5 * fma(0.95, uniform53(a.seed, (uint64_t)c, lane,
+                                                                         0, 2, a.counter), 0.05));
+                    q[CENTER][1] = (float)(q1 + 0.5 * fma(0.95, uniform53(a.seed, (uint64_t)c, lane,
+                                                                         1, 2, a.counter), 0.05));
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        const double v = xv[d] + dq[d] * (double)((qd >> d) & 1);
+                        const double r = uniform53(a.seed, (uint64_t)c, lane, d, 2, a.counter);
+                        q[CENTER][d] = v + dq[d] * fma(0.95, r, 0.05);
+                    }
+                }
+            }
+        }
+        // the searches, in injection (= ascending slot) order
+        double fc = f[CENTER];
+        for (uint64_t bits = injm; bits; bits &= bits - 1) {
+            const int ip = __builtin_ctzll(bits);
+            P pn[2];
+            if constexpr (SPLIT) {
+                pn[0] = readlane_f(q[CENTER][0], ip);
+                pn[1] = readlane_f(q[CENTER][1], ip);
+            } else {
+                pn[0] = readlane_d(q[CENTER][0], ip);
+                pn[1] = readlane_d(q[CENTER][1], ip);
+            }
+            // this cell's candidates: its particles before the call and the
+            // injections before this one
+            const bool cand_c = (act0 || (((injm >> lane) & 1) && lane < ip)) && lane != ip;
+            double best = __builtin_inf();
+            int border = 0x7fffffff;
+            double s9[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                s9[nb] = __builtin_inf();
+                const bool av = nb == CENTER ? cand_c : (bool)((o9[nb] >> lane) & 1);
+                if (!av) continue;
+                double s;
+                if constexpr (SPLIT) {
+                    const double r0 =
+                        ((double)(nb % 3 - 1) + (double)q[nb][0] - (double)pn[0]) * a.h[0];
+                    const double r1 =
+                        ((double)(nb / 3 - 1) + (double)q[nb][1] - (double)pn[1]) * a.h[1];
+                    s = r0 * r0 + r1 * r1;
+                } else {
+                    if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
+                    const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
+                    s = du * du + dv * dv;
+                }
+                s9[nb] = s;
+                if (s < best) {
+                    best = s;
+                    border = nb * 64 + lane;
+                }
+            }
+            wave_argmin(best, border);
+            resolve_ties(s9, best, border, lane);
+            if (border == 0x7fffffff) {
+                ++n_orph;  // no neighbour at all: the field keeps its slot's value
+            } else if (has_arg) {
+                const int nbw = border >> 6, lw = border & 63;  // uniform
+                double src = fc;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    if (nb != CENTER && nb == nbw) src = f[nb];
+                const double v = readlane_d(src, lw);
+                if (lane == ip) fc = v;
+            }
+        }
+        n_inj += __popcll(injm);
+        if ((injm >> lane) & 1) {
+            const int64_t e = c + C * lane;
+            if constexpr (SPLIT) {
+                a.u[0][e] = q[CENTER][0];
+                a.u[1][e] = q[CENTER][1];
+            } else {
+                a.x[0][e] = q[CENTER][0];
+                a.x[1][e] = q[CENTER][1];
+            }
+            a.index[e] = 1;
+            if (has_arg) a.arg[e] = fc;
+        }
+        // later colour phases see this cell's final occupancy
+        if (lane == 0) a.occw[c] = o9[CENTER] | injm;
+    }  // work list
+    if (lane == 0) {
+        stat_add(a.st, ST_INJECTED, n_inj);
+        stat_add(a.st, ST_ORPHAN, n_orph);
+    }
+}
+
+}  // namespace jp

@@ -294,23 +294,26 @@ __device__ inline int64_t nb_cell(const Grid<D>& g, const int64_t* I, int n, boo
     return ok ? lin<D>(g.ni, J) : -1;
 }
 
+// Per thread in LDS: for each cell of the 3^D neighbourhood its occupancy
+// word and cell number (one 16-byte entry), and the cell's code words.  The
+// mover loop is one serial chain per thread with under two wavefronts per
+// SIMD, so it is kept short: per mover one LDS byte read (the code), then one
+// read of the destination's entry, 32-bit slot arithmetic (a move has fewer
+// than 2^32 slots), and this cell's own word in a register.
+struct alignas(16) NbEntry {
+    uint64_t occ;
+    uint32_t cell;  // 0xFFFFFFFF outside the grid (never a destination)
+    uint32_t pad;
+};
+
+// Colour cell t of phase `phase` (offsets `off`): the reference slot loop of
+// that cell.  `stop`: the serial fallback runs instead (checked after the
+// cell's loads are issued).
 template <int D, int BS>
-__global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3 off, int phase,
-                                                 const int* __restrict__ flag, jp_stats* st) {
-    // Per thread in LDS: for each cell of the 3^D neighbourhood its occupancy
-    // word and cell number (one 16-byte entry), and the cell's code words.
-    // The mover loop is one serial chain per thread with under two wavefronts
-    // per SIMD, so it is kept short: per mover one LDS byte read (the code),
-    // then one read of the destination's entry, 32-bit slot arithmetic (a move
-    // has fewer than 2^32 slots), and this cell's own word in a register.
-    struct alignas(16) NbEntry {
-        uint64_t occ;
-        uint32_t cell;  // 0xFFFFFFFF outside the grid (never a destination)
-        uint32_t pad;
-    };
-    __shared__ NbEntry s_nb[kNB<D>][BS];
-    __shared__ uint64_t s_cw[8][BS];
-    const int stop = *flag;  // checked below, after this cell's loads are issued
+__device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const Off3& off,
+                                  int phase, int64_t t, bool stop, NbEntry (&s_nb)[kNB<D>][BS],
+                                  uint64_t (&s_cw)[8][BS], int& n_moved, int& n_del,
+                                  int& n_drop) {
     int64_t ncol[D];
     int64_t total = 1;
 #pragma unroll
@@ -318,7 +321,6 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
         ncol[d] = (g.ni[d] + 2) / 3;
         total *= ncol[d];
     }
-    int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
     if (t >= total) return;
     int64_t I[D];
     bool ok = true;
@@ -374,7 +376,6 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     uint64_t selfm = 0;  // slots that received a re-slotted particle of this cell
     uint64_t selfins = 0;
     int sp = 0;
-    int n_moved = 0, n_del = 0, n_drop = 0;
     while (pend) {
         const int ip = __builtin_ctzll(pend);
         pend &= pend - 1;
@@ -461,6 +462,18 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     ms.arr[S] = arrS;
     ms.tch[S] = tchS | selfins;
     ms.mov[S] = 0;
+}
+
+// one colour phase, one thread per colour cell
+template <int D, int BS>
+__global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3 off, int phase,
+                                                 const int* __restrict__ flag, jp_stats* st) {
+    __shared__ NbEntry s_nb[kNB<D>][BS];
+    __shared__ uint64_t s_cw[8][BS];
+    const int stop = *flag;
+    int n_moved = 0, n_del = 0, n_drop = 0;
+    phase_cell<D, BS>(g, ms, off, phase, blockIdx.x * (int64_t)BS + threadIdx.x, stop != 0, s_nb,
+                      s_cw, n_moved, n_del, n_drop);
     stat_add(st, ST_MOVED, n_moved);
     stat_add(st, ST_DELETED, n_del);
     stat_add(st, ST_DROPPED, n_drop);
@@ -885,7 +898,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     } else {
         if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
         k_mv_classify_split<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
-                                                                 flag, span, dgeom(ctx));
+                                                                 flag, span, geo_ptr(ctx));
     }
     JP_LAUNCHED(ctx);
     int64_t total = 1;

@@ -10,8 +10,10 @@
 // the centre axis extended by one ghost point on each side elsewhere
 // (README.md:42-44).
 #include "jp_ctx.hpp"
+#include "jp_inject2.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -79,7 +81,8 @@ __device__ double axis_deviation(const Axis& a, bool with_dx, double& h, double*
 // of them is uniform, and derives their affine maps (SplitGeom).
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_split_geom(Grid<D> g, Vel<D> v, int with_vel,
-                                                       SplitGeom* __restrict__ geo) {
+                                                       SplitGeom* __restrict__ geo,
+                                                       int32_t* __restrict__ sticky) {
     __shared__ double red[kBlock / 64];
     int bad = 0;
     double h[D];
@@ -104,7 +107,7 @@ __global__ void __launch_bounds__(kBlock) k_split_geom(Grid<D> g, Vel<D> v, int 
         }
     if (threadIdx.x == 0) {
         geo->bad = bad;
-        geo->sticky |= bad;
+        *sticky |= bad;
 #pragma unroll
         for (int c = 0; c < D; ++c)
 #pragma unroll
@@ -200,12 +203,20 @@ struct CellMap {
     float r[D][D];
 };
 
-template <int D>
+// STD: the standard staggering, r = 1 and o = 0 (the component's own axis) or
+// 1/2 (a centre axis with one ghost node), as compile-time constants
+template <int D, bool STD = false>
 __device__ inline void cell_map(const SMap<D>& m, const int (&I)[D], CellMap<D>& cm) {
 #pragma unroll
     for (int c = 0; c < D; ++c)
 #pragma unroll
         for (int d = 0; d < D; ++d) {
+            if constexpr (STD) {
+                cm.ei[c][d] = 0;
+                cm.ef[c][d] = c == d ? 0.0f : 0.5f;
+                cm.r[c][d] = 1.0f;
+                continue;
+            }
             const double e = fma((double)I[d], m.r[c][d] - 1.0, m.o[c][d]);
             const double fl = floor(e);
             cm.ei[c][d] = (int)fl;
@@ -215,8 +226,9 @@ __device__ inline void cell_map(const SMap<D>& m, const int (&I)[D], CellMap<D>&
 }
 
 // first node of the LDS window of component c along d for a tile at t0
-template <int D>
+template <int D, bool STD = false>
 __device__ inline int window_origin(const SMap<D>& m, int c, int d, int t0) {
+    if constexpr (STD) return t0 - 1;
     return t0 + (int)floor(fma((double)t0, m.r[c][d] - 1.0, m.o[c][d])) - 1;
 }
 
@@ -335,7 +347,7 @@ __device__ inline void advect_split_one(const float (*sV)[TS::size()], const SVe
 
 // advection of the particles in slots kb..ke-1 of tile `tile` of one grid (all
 // threads of the block call it; the LDS window sV is the block's)
-template <int D, int M, class TS>
+template <int D, int M, class TS, bool STD = false>
 __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, const SMap<D>& m,
                                       double dt, const FPtr3& off,
                                       const uint8_t* __restrict__ index, const RKf& P, int tile,
@@ -357,7 +369,7 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, con
 #pragma unroll
     for (int c = 0; c < D; ++c)
 #pragma unroll
-        for (int d = 0; d < D; ++d) w0[c][d] = window_origin<D>(m, c, d, t0[d]);
+        for (int d = 0; d < D; ++d) w0[c][d] = window_origin<D, STD>(m, c, d, t0[d]);
     const int tid = threadIdx.x;
     // window fill: every load issued before the LDS stores
     constexpr int NI = (TS::size() + kBlock - 1) / kBlock;
@@ -401,7 +413,7 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, con
     }
     if (!in) return;
     CellMap<D> cm;
-    cell_map<D>(m, I, cm);
+    cell_map<D, STD>(m, I, cm);
     int64_t c = I[D - 1];
 #pragma unroll
     for (int d = D - 2; d >= 0; --d) c = c * g.ni[d] + I[d];
@@ -450,7 +462,8 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, con
 
 // blockIdx.y: slot group of `span` slots (a multiple of 8), so that grids with
 // few tiles still fill the chip; each group stages its own velocity window
-template <int D, int M>
+// STD: axes accepted with the standard staggering (the maps are constants)
+template <int D, int M, bool STD>
 __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V,
                                                       const SplitGeom* __restrict__ geo, double dt,
                                                       FPtr3 off, const uint8_t* __restrict__ index,
@@ -463,12 +476,12 @@ __global__ void __launch_bounds__(kBlock) k_adv_split(SGrid<D> g, SVelIn<D> V,
     for (int c = 0; c < D; ++c)
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            m.r[c][d] = geo->r[c][d];
-            m.o[c][d] = geo->o[c][d];
+            m.r[c][d] = STD ? 1.0 : geo->r[c][d];
+            m.o[c][d] = STD ? (c == d ? 0.0 : 0.5) : geo->o[c][d];
         }
     const int kb = blockIdx.y * span;
     const int ke = kb + span < g.K ? kb + span : g.K;
-    adv_split_tile<D, M, TS>(g, V, m, dt, off, index, P, blockIdx.x, kb, ke, sV);
+    adv_split_tile<D, M, TS, STD>(g, V, m, dt, off, index, P, blockIdx.x, kb, ke, sV);
 }
 
 // ------------------------------------------------------------ ensemble
@@ -722,7 +735,7 @@ __global__ void __launch_bounds__(kBlock) k_inj_scan_split(Grid<D> g, FPtr3 off,
             }
         }
         L.occ[c] = act;
-        if (D == 2) {
+        if (D == 2 && L.box) {
             float* __restrict__ bx = static_cast<float*>(L.box);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -883,227 +896,130 @@ __global__ void __launch_bounds__(kBlock) k_inj_wave_split(Grid<D> g, SGrid<D> s
     }
 }
 
-// 2D with at most one field, loading only what the nearest-neighbour search
-// can use: the cell's own particles, and a neighbour cell's particles only
-// when the distance from the new particle to the box of that cell's offsets
-// (same floating point operations as the distances) does not exceed the best
-// distance so far.  Loaded neighbours stay in registers for the cell's later
-// injections; copies come from registers or one uniform load.
-__global__ void __launch_bounds__(kBlock) k_inj_wave2_split(Grid<2> g, SGrid<2> sg, FPtr3 off,
-                                                            uint8_t* index, int nargs,
-                                                            double* __restrict__ arg, int min_q,
-                                                            uint64_t seed, uint64_t counter,
-                                                            const int32_t* __restrict__ list,
-                                                            const int32_t* __restrict__ count,
-                                                            uint64_t* __restrict__ occw,
-                                                            float* __restrict__ box,
-                                                            jp_stats* st) {
-    constexpr int NB = 9, CENTER = 4;
-    const int lane = threadIdx.x & 63;
-    const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
-    const int64_t total = *count;
-    const int64_t C = g.C;
-    const int K = g.K;
-    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
-    const double h0 = sg.h[0][0], h1 = sg.h[1][0];
-    int64_t n_inj = 0, n_orph = 0;
-    for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
-         w += nwv) {
-        const int64_t c = __builtin_amdgcn_readfirstlane(list[w]);
-        int64_t I[2];
-        unlin<2>(g.ni, c, I);
-        int64_t cn[NB];
-        uint64_t o9[NB];
-        float bl[NB][2], bh[NB][2];
-        bool okn[NB];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
-            okn[nb] = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
-            cn[nb] = okn[nb] ? J0 + g.ni[0] * J1 : c;  // a valid cell either way
-        }
-        // unguarded loads, all issued before any is used; masked afterwards
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            o9[nb] = occw[cn[nb]];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                bl[nb][d] = box[(2 * d) * C + cn[nb]];
-                bh[nb][d] = box[(2 * d + 1) * C + cn[nb]];
-            }
-        }
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
-        const int64_t e = c + C * lane;
-        bool act = (o9[CENTER] >> lane) & 1;
-        float uq[NB][2];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) uq[nb][0] = uq[nb][1] = 0.0f;
-        uq[CENTER][0] = act ? off.p[0][e] : 0.0f;
-        uq[CENTER][1] = act ? off.p[1][e] : 0.0f;
-        double a_c = (act && nargs) ? arg[e] : 0.0;
-        // this cell's new particles stay in registers until the cell is done
-        // (a load issued after a store would also wait for the store)
-        bool inj = false, copied = false;
-        uint32_t loaded = 1u << CENTER;
-        for (int qd = 0; qd < 4; ++qd) {
-            const float q0 = 0.5f * (float)(qd & 1), q1 = 0.5f * (float)((qd >> 1) & 1);
-            const bool inside = act && (q0 < uq[CENTER][0] && uq[CENTER][0] < q0 + 0.5f) &&
-                                (q1 < uq[CENTER][1] && uq[CENTER][1] < q1 + 0.5f);
-            int num = __popcll(__ballot(inside));
-            if (num >= min_q) break;
-            uint64_t freeS = ~__ballot(act) & full;
-            while (freeS) {
-                const int ip = __builtin_ctzll(freeS);
-                freeS &= freeS - 1;
-                ++num;
-                float un[2];
-                un[0] = (float)((double)q0 +
-                                0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 0, 2, counter), 0.05));
-                un[1] = (float)((double)q1 +
-                                0.5 * fma(0.95, uniform53(seed, (uint64_t)c, ip, 1, 2, counter), 0.05));
-                bl[CENTER][0] = fminf(bl[CENTER][0], un[0]);
-                bh[CENTER][0] = fmaxf(bh[CENTER][0], un[0]);
-                bl[CENTER][1] = fminf(bl[CENTER][1], un[1]);
-                bh[CENTER][1] = fmaxf(bh[CENTER][1], un[1]);
-                if (lane == ip) {
-                    act = true;
-                    inj = true;
-                    uq[CENTER][0] = un[0];
-                    uq[CENTER][1] = un[1];
-                }
-                double best;
-                int border;
-                // best holds the smallest SQUARED distance; near ties of the
-                // rounded distance are resolved exactly (as in the FP64 kernel)
-                auto search = [&]() __attribute__((always_inline)) {
-                    best = __builtin_inf();
-                    border = 0x7fffffff;
-                    double s9[NB];
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) {
-                        s9[nb] = __builtin_inf();
-                        if (!((loaded >> nb) & 1)) continue;
-                        const bool av = nb == CENTER ? (act && lane != ip) : ((o9[nb] >> lane) & 1);
-                        if (!av) continue;
-                        const double r0 =
-                            ((double)(nb % 3 - 1) + (double)uq[nb][0] - (double)un[0]) * h0;
-                        const double r1 =
-                            ((double)(nb / 3 - 1) + (double)uq[nb][1] - (double)un[1]) * h1;
-                        s9[nb] = r0 * r0 + r1 * r1;
-                        if (s9[nb] < best) {
-                            best = s9[nb];
-                            border = nb * 64 + lane;
-                        }
-                    }
-                    wave_argmin(best, border);
-                    resolve_ties(s9, best, border, lane);
-                };
-                search();
-                uint32_t need = 0;
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    if (nb == CENTER || ((loaded >> nb) & 1) || !o9[nb]) continue;
-                    // r(u) = ((o + u) - un) * h is monotonic in u: its smallest
-                    // magnitude over [lo, hi] bounds every particle of the cell
-                    const double lo0 = ((double)(nb % 3 - 1) + (double)bl[nb][0] - (double)un[0]) * h0;
-                    const double hi0 = ((double)(nb % 3 - 1) + (double)bh[nb][0] - (double)un[0]) * h0;
-                    const double lo1 = ((double)(nb / 3 - 1) + (double)bl[nb][1] - (double)un[1]) * h1;
-                    const double hi1 = ((double)(nb / 3 - 1) + (double)bh[nb][1] - (double)un[1]) * h1;
-                    const double gx = lo0 > 0.0 ? lo0 : (hi0 < 0.0 ? hi0 : 0.0);
-                    const double gy = lo1 > 0.0 ? lo1 : (hi1 < 0.0 ? hi1 : 0.0);
-                    if (!(gx * gx + gy * gy > best * kPrune)) need |= 1u << nb;
-                }
-                if (need) {
-                    // one predicate per lane (no uniform branch per cell), so the
-                    // loads of all needed cells are in flight together
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) {
-                        if (nb == CENTER) continue;
-                        const bool take = ((need >> nb) & 1) && ((o9[nb] >> lane) & 1);
-                        const int64_t e2 = cn[nb] + C * lane;
-                        uq[nb][0] = take ? off.p[0][e2] : uq[nb][0];
-                        uq[nb][1] = take ? off.p[1][e2] : uq[nb][1];
-                    }
-                    loaded |= need;
-                    search();
-                }
-                if (border == 0x7fffffff) {
-                    ++n_orph;
-                } else if (nargs) {
-                    const int nbw = border >> 6, lw = border & 63;
-                    double v;
-                    if (nbw == CENTER) {
-                        v = __shfl(a_c, lw, 64);
-                    } else {
-                        int64_t cs = cn[0];
-#pragma unroll
-                        for (int nb = 1; nb < NB; ++nb) cs = nb == nbw ? cn[nb] : cs;
-                        v = arg[cs + C * lw];
-                    }
-                    if (lane == ip) {
-                        a_c = v;
-                        copied = true;
-                    }
-                }
-                ++n_inj;
-                if (num >= min_q) break;
-            }
-        }
-        if (inj) {
-            off.p[0][e] = uq[CENTER][0];
-            off.p[1][e] = uq[CENTER][1];
-            index[e] = 1;
-            if (copied) arg[e] = a_c;
-        }
-        // later colour phases see this cell's final occupancy and box
-        const uint64_t fin = __ballot(act) & full;
-        if (lane == 0) {
-            occw[c] = fin;
-            box[c] = bl[CENTER][0];
-            box[C + c] = bh[CENTER][0];
-            box[2 * C + c] = bl[CENTER][1];
-            box[3 * C + c] = bh[CENTER][1];
-        }
-    }  // work list
-    if (lane == 0) {
-        stat_add(st, ST_INJECTED, n_inj);
-        stat_add(st, ST_ORPHAN, n_orph);
-    }
-}
-
 // ------------------------------------------------------------ host side
 }  // namespace
 
 namespace jp {
 
-int split_geometry(jp_ctx* ctx, const jp_cellgrid* gc, const jp_velgrid* vg) {
+namespace {
+int geom_reject(jp_ctx* ctx, int bits);
+
+template <int D>
+void launch_geom(jp_ctx* ctx, const jp_cellgrid* gc, const jp_velgrid* vg, SplitGeom* slot) {
+    Grid<D> G = to_grid<D>(gc);
+    Vel<D> v{};
     if (vg)
-        for (int c = 0; c < gc->ndim; ++c)
-            for (int d = 0; d < gc->ndim; ++d)
+        for (int c = 0; c < D; ++c)
+            for (int d = 0; d < D; ++d)
+                v.ax[c][d] = Axis{vg->ax[c][d].x, vg->ax[c][d].dx, vg->ax[c][d].inv_dx,
+                                  vg->ax[c][d].n};
+    k_split_geom<D><<<1, kBlock, 0, ctx->stream>>>(G, v, vg != nullptr, slot,
+                                                   &dgeom(ctx)->sticky);
+}
+}  // namespace
+
+int split_geometry(jp_ctx* ctx, const jp_cellgrid* gc, const jp_velgrid* vg) {
+    const int D = gc->ndim;
+    if (vg)
+        for (int c = 0; c < D; ++c)
+            for (int d = 0; d < D; ++d)
                 if (!vg->ax[c][d].x || vg->ax[c][d].n < 2 || vg->ax[c][d].n > (1 << 30))
                     return fail(ctx, JP_ERR_ARG, "velocity axis%s %lld needs at least two points",
                                 "", c * 3 + d);
-    if (gc->ndim == 2) {
-        Grid<2> G = to_grid<2>(gc);
-        Vel<2> v{};
-        if (vg)
-            for (int c = 0; c < 2; ++c)
-                for (int d = 0; d < 2; ++d)
-                    v.ax[c][d] = Axis{vg->ax[c][d].x, vg->ax[c][d].dx, vg->ax[c][d].inv_dx,
-                                      vg->ax[c][d].n};
-        k_split_geom<2><<<1, kBlock, 0, ctx->stream>>>(G, v, vg != nullptr, dgeom(ctx));
-    } else {
-        Grid<3> G = to_grid<3>(gc);
-        Vel<3> v{};
-        if (vg)
-            for (int c = 0; c < 3; ++c)
-                for (int d = 0; d < 3; ++d)
-                    v.ax[c][d] = Axis{vg->ax[c][d].x, vg->ax[c][d].dx, vg->ax[c][d].inv_dx,
-                                      vg->ax[c][d].n};
-        k_split_geom<3><<<1, kBlock, 0, ctx->stream>>>(G, v, vg != nullptr, dgeom(ctx));
+    GeomKey key;
+    std::memset(&key, 0, sizeof key);
+    key.ndim = D;
+    key.with_vel = vg != nullptr;
+    for (int d = 0; d < D; ++d) {
+        key.ni[d] = gc->ni[d];
+        key.xv[d] = gc->xv[d];
     }
+    if (vg)
+        for (int c = 0; c < D; ++c)
+            for (int d = 0; d < D; ++d) key.vel[c][d] = vg->ax[c][d];
+    const uint64_t gen = free_generation();
+    // FNV-1a over the key's bytes
+    uint64_t hsh = 1469598103934665603ull;
+    {
+        const unsigned char* b = reinterpret_cast<const unsigned char*>(&key);
+        for (size_t i = 0; i < sizeof key; ++i) hsh = (hsh ^ b[i]) * 1099511628211ull;
+    }
+    int slot = -1;
+    {
+        auto it = ctx->geo_index.find(hsh);
+        if (it != ctx->geo_index.end()) {
+            const GeomEntry& e = ctx->geo_cache[it->second];
+            if (e.state != 0 && e.gen == gen && std::memcmp(&e.key, &key, sizeof key) == 0)
+                slot = it->second;
+        }
+    }
+    if (slot >= 0 && !ctx->opt_geom_recheck && ctx->geo_cache[slot].state == 1) {
+        // accepted before: no check; a captured graph keeps reading this slot
+        GeomEntry& e = ctx->geo_cache[slot];
+        ctx->geo_cur = slot;
+        e.used = ++ctx->geo_clock;
+        if (ctx->capturing) e.pinned = true;
+        return JP_OK;
+    }
+    if (slot < 0) {
+        // an empty slot, else the least recently used one no graph reads
+        slot = -1;
+        for (int i = 0; i < kGeomSlots; ++i) {
+            const GeomEntry& a = ctx->geo_cache[i];
+            if (a.pinned) continue;
+            if (a.state == 0) {
+                slot = i;
+                break;
+            }
+            if (slot < 0 || a.used < ctx->geo_cache[slot].used) slot = i;
+        }
+        if (slot < 0)
+            return fail(ctx, JP_ERR_STATE,
+                        "every split geometry slot is read by a captured graph%s (%lld)", "",
+                        kGeomSlots);
+        const GeomEntry& old = ctx->geo_cache[slot];
+        if (old.state != 0) {
+            uint64_t oh = 1469598103934665603ull;
+            const unsigned char* b = reinterpret_cast<const unsigned char*>(&old.key);
+            for (size_t i = 0; i < sizeof old.key; ++i) oh = (oh ^ b[i]) * 1099511628211ull;
+            auto it = ctx->geo_index.find(oh);
+            if (it != ctx->geo_index.end() && it->second == slot) ctx->geo_index.erase(it);
+        }
+    }
+    ctx->geo_index[hsh] = slot;
+    GeomEntry& e = ctx->geo_cache[slot];
+    SplitGeom* dev = static_cast<SplitGeom*>(ctx->geo_dev) + slot;
+    if (D == 2)
+        launch_geom<2>(ctx, gc, vg, dev);
+    else
+        launch_geom<3>(ctx, gc, vg, dev);
     JP_LAUNCHED(ctx);
+    e.key = key;
+    e.gen = gen;
+    e.used = ++ctx->geo_clock;
+    e.standard = false;
+    ctx->geo_cur = slot;
+    if (ctx->capturing) {
+        // the verdict stays on the device (the graph's kernels check it); the
+        // graph writes and reads this slot, which no other axes take over (a
+        // later check of the same axes writes the same values)
+        e.state = 2;
+        e.pinned = true;
+        return JP_OK;
+    }
+    SplitGeom h;
+    JP_HIP(ctx, hipMemcpyAsync(&h, dev, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h.bad) {
+        e.state = 0;
+        return geom_reject(ctx, h.bad);
+    }
+    e.state = 1;
+    bool standard = vg != nullptr;
+    for (int c = 0; c < D && standard; ++c)
+        for (int d = 0; d < D; ++d)
+            standard = standard && h.r[c][d] == 1.0 && h.o[c][d] == (c == d ? 0.0 : 0.5);
+    e.standard = standard;
     return JP_OK;
 }
 
@@ -1127,9 +1043,9 @@ int geom_reject_now(jp_ctx* ctx, int bits) { return geom_reject(ctx, bits); }
 
 // checks this call's geometry verdict now, whatever the synchronisation mode
 int split_finish_sync(jp_ctx* ctx) {
-    if (ctx->capturing) return JP_OK;
+    if (ctx->capturing || ctx->geo_cache[ctx->geo_cur].state == 1) return JP_OK;
     int32_t bad = 0;
-    JP_HIP(ctx, hipMemcpyAsync(&bad, &dgeom(ctx)->bad, sizeof bad, hipMemcpyDeviceToHost,
+    JP_HIP(ctx, hipMemcpyAsync(&bad, &geo_ptr(ctx)->bad, sizeof bad, hipMemcpyDeviceToHost,
                                ctx->stream));
     JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return bad ? geom_reject(ctx, bad) : JP_OK;
@@ -1137,8 +1053,9 @@ int split_finish_sync(jp_ctx* ctx) {
 
 int split_finish(jp_ctx* ctx) {
     if (!ctx->sync || ctx->capturing) return JP_OK;
+    if (ctx->geo_cache[ctx->geo_cur].state == 1) return finish(ctx);  // accepted axes
     int32_t bad = 0;
-    JP_HIP(ctx, hipMemcpyAsync(&bad, &dgeom(ctx)->bad, sizeof bad, hipMemcpyDeviceToHost,
+    JP_HIP(ctx, hipMemcpyAsync(&bad, &geo_ptr(ctx)->bad, sizeof bad, hipMemcpyDeviceToHost,
                                ctx->stream));
     JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return bad ? geom_reject(ctx, bad) : JP_OK;
@@ -1185,7 +1102,7 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     }
     int rc = split_geometry(ctx, gc, vg);
     if (rc) return rc;
-    const SplitGeom* geo = dgeom(ctx);
+    const SplitGeom* geo = geo_ptr(ctx);
     RKf P;
     P.alpha = (float)alpha;
     P.half = alpha == 0.5;
@@ -1198,15 +1115,27 @@ int run_adv(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, const uint8_t
     FPtr3 fp = fptr3<D>(off);
     const int span = slot_span(ctx, tiles, G.K);
     const dim3 grid((unsigned)tiles, (unsigned)((G.K + span - 1) / span));
-    if (method == JP_EULER)
-        k_adv_split<D, JP_EULER><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp, index, P,
-                                                                   span);
-    else if (method == JP_RK2)
-        k_adv_split<D, JP_RK2><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp, index, P,
-                                                                 span);
-    else
-        k_adv_split<D, JP_RK4><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp, index, P,
-                                                                 span);
+    if (geo_standard(ctx)) {
+        if (method == JP_EULER)
+            k_adv_split<D, JP_EULER, true><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp,
+                                                                             index, P, span);
+        else if (method == JP_RK2)
+            k_adv_split<D, JP_RK2, true><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp,
+                                                                           index, P, span);
+        else
+            k_adv_split<D, JP_RK4, true><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp,
+                                                                           index, P, span);
+    } else {
+        if (method == JP_EULER)
+            k_adv_split<D, JP_EULER, false><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp,
+                                                                              index, P, span);
+        else if (method == JP_RK2)
+            k_adv_split<D, JP_RK2, false><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp,
+                                                                            index, P, span);
+        else
+            k_adv_split<D, JP_RK4, false><<<grid, kBlock, 0, ctx->stream>>>(sg, sv, geo, dt, fp,
+                                                                            index, P, span);
+    }
     JP_LAUNCHED(ctx);
     return split_finish(ctx);
 }
@@ -1226,7 +1155,7 @@ int run_p2g(jp_ctx* ctx, const jp_cellgrid* gc, double* F, const double* Fp, flo
         G, sg, Fp, fptr3<D>(off), index, part);
     JP_LAUNCHED(ctx);
     k_p2g_split_nodes<D><<<(unsigned)blocks_for(NN), kBlock, 0, ctx->stream>>>(G, sg, part,
-                                                                              dgeom(ctx), F);
+                                                                              geo_ptr(ctx), F);
     JP_LAUNCHED(ctx);
     return split_finish(ctx);
 }
@@ -1254,11 +1183,11 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     int64_t cap = 1;
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
-    if ((rc = inject_lists(ctx, G.C, NQ, cap, D == 2 ? (size_t)G.C * 4 * 2 * D : 0, L))) return rc;
+    if ((rc = inject_lists(ctx, G.C, NQ, cap, 0, L))) return rc;
     FPtr3 fp = fptr3<D>(off);
     if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
     k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, L,
-                                                                     dgeom(ctx));
+                                                                     geo_ptr(ctx));
     JP_LAUNCHED(ctx);
     if ((rc = join_segments(ctx, NQ, L))) return rc;
     ArgPtrs ap;
@@ -1267,10 +1196,22 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     for (int ph = 0; ph < NQ; ++ph) {
         if constexpr (D == 2) {
             if (nargs <= 1) {
-                k_inj_wave2_split<<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
-                    G, sg, fp, index, nargs, nargs ? args[0] : nullptr, min_q, seed, counter,
-                    L.list + (int64_t)ph * cap, L.total + ph, L.occ,
-                    static_cast<float*>(L.box), st);
+                Inject2 a{};
+                a.g = G;
+                a.u[0] = off[0];
+                a.u[1] = off[1];
+                a.h[0] = sg.h[0][0];
+                a.h[1] = sg.h[1][0];
+                a.index = index;
+                a.arg = nargs ? args[0] : nullptr;
+                a.min_q = min_q;
+                a.seed = seed;
+                a.counter = counter;
+                a.list = L.list + (int64_t)ph * cap;
+                a.count = L.total + ph;
+                a.occw = L.occ;
+                a.st = st;
+                k_inject2<true><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(a);
                 JP_LAUNCHED(ctx);
                 continue;
             }
@@ -1485,7 +1426,7 @@ int jp_ensemble_create(jp_ctx* ctx, int n, const jp_cellgrid* grids, const jp_ve
         // axis maps of this grid, checked now (the ensemble keeps its arrays)
         if ((rc = split_geometry(ctx, g, vgs + i))) return rc;
         SplitGeom geo;
-        JP_HIP(ctx, hipMemcpyAsync(&geo, dgeom(ctx), sizeof geo, hipMemcpyDeviceToHost,
+        JP_HIP(ctx, hipMemcpyAsync(&geo, geo_ptr(ctx), sizeof geo, hipMemcpyDeviceToHost,
                                    ctx->stream));
         JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (geo.bad) return geom_reject_now(ctx, geo.bad);

@@ -235,3 +235,4 @@ def test_split_loop_with_inject_and_p2g(ctx):
     assert np.array_equal(np.isnan(F.flat()), np.isnan(Fo))
     # float32 weights (relative error ~1e-7 of each weight) on values in [0, 1]
     assert np.nanquantile(dF, 0.999) <= 1e-5
+

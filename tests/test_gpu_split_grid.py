@@ -227,17 +227,69 @@ def test_split_entry_points_reject_nonuniform_axes_in_c(ctx):
     rc = calls["advection (vertex)"](g_ok, vbad)
     assert rc == _abi.JP_ERR_ARG and "uniform velocity" in lib.jp_last_error(h).decode()
     assert _same(_state(s), before)
-    # asynchronous: the call returns 0, the next jp_sync reports the rejection
+    # asynchronous: axes are checked (and read back) on first use, so the
+    # rejection is immediate there too
     ctx.set_sync(False)
     try:
         rc = calls["advection (vertex)"](g_ok, vbad)
-        assert rc == 0
-        rc = lib.jp_sync(h)
         assert rc == _abi.JP_ERR_ARG and "uniform velocity" in lib.jp_last_error(h).decode()
-        assert lib.jp_sync(h) == 0, "the report clears the sticky word"
+        assert lib.jp_sync(h) == 0
     finally:
         ctx.set_sync(True)
+    assert _same(_state(s), before)
+    # under graph capture the verdict stays on the device: the captured call
+    # returns 0, its kernels skip all work, and jp_sync after the replay
+    # reports the rejection once
+    ctx.graph_begin()
+    rc = calls["advection (vertex)"](g_ok, vbad)
+    graph = ctx.graph_end()
+    assert rc == 0
+    ctx.graph_launch(graph)
+    rc = lib.jp_sync(h)
+    assert rc == _abi.JP_ERR_ARG and "uniform velocity" in lib.jp_last_error(h).decode()
+    assert lib.jp_sync(h) == 0, "the report clears the sticky word"
+    ctx.graph_destroy(graph)
     assert _same(_state(s), before)
     # the good grid still advects
     jp.advection_(s, jp.RungeKutta2(), Vd, cfg["dt"])
     assert not _same(_state(s), before)
+
+
+def test_geometry_verdict_cached_until_free_or_recheck(ctx):
+    """Axes are checked once per set of pointers: a vertex axis made
+    non-uniform in place afterwards is still trusted (the documented cost of
+    the cache), JP_OPT_GEOMETRY_RECHECK = 1 checks every call and rejects it,
+    and any jp_free drops every cached verdict."""
+    cfg = stream2d(32)
+    p = jp.init_particles(ctx, 8, 16, 6, *cfg["xi_vel"], seed=4)
+    s = jp.to_split(p)
+    Vd = dev_V(ctx, cfg["V"])
+    lib, h = ctx.lib, ctx.h
+    keep = []
+    x = cfg["xv"][0].x.copy()
+    g = _abi.jp_cellgrid.from_buffer_copy(s._grid)
+    g.xv[0] = _dev_axis(ctx, x, keep)
+    Vp = _abi.ptr_array([v.ptr for v in Vd])
+
+    def advect():
+        return lib.jp_advection_split(h, C.byref(g), s.coord_ptrs(), s.index.ptr, Vp,
+                                      C.byref(s._velgrid), _abi.JP_RK2, 0.5, cfg["dt"])
+
+    assert advect() == 0
+    xb = x.copy()
+    xb[7] += 1e-6 * (xb[1] - xb[0])
+    keep[0].copy_from(xb)  # in place: same pointer, other coordinates
+    assert advect() == 0  # cached verdict
+    ctx.set_option(_abi.JP_OPT_GEOMETRY_RECHECK, 1)
+    try:
+        before = _state(s)
+        assert advect() == _abi.JP_ERR_ARG
+        assert "uniform vertex" in lib.jp_last_error(h).decode()
+        assert _same(_state(s), before)
+    finally:
+        ctx.set_option(_abi.JP_OPT_GEOMETRY_RECHECK, 0)
+    keep[0].copy_from(x)
+    assert advect() == 0
+    keep[0].copy_from(xb)
+    jp.DeviceArray(ctx, (4,)).free()  # any jp_free drops the cached verdicts
+    assert advect() == _abi.JP_ERR_ARG
