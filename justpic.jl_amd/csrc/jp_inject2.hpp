@@ -96,11 +96,12 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
         for (int nb = 0; nb < NB; ++nb) o9[nb] = a.occw[cn[nb]];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
-        // the whole neighbourhood in one round of loads: positions of the
-        // active slots, field values (this cell's for every slot: an orphan
-        // injection keeps the value its slot holds)
+        // positions of the neighbourhood's particles (active slots only: in
+        // the slot-plane layout every slot is its own cache line) and this
+        // cell's field values (every slot: an orphan injection keeps the value
+        // its slot holds); a neighbour's value is read only when a search
+        // picks it, below
         P q[NB][2];
-        double f[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const bool take = (o9[nb] >> lane) & 1;
@@ -112,15 +113,21 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
                 q[nb][0] = take ? a.x[0][e2] : 0.0;
                 q[nb][1] = take ? a.x[1][e2] : 0.0;
             }
-            const bool takef = has_arg && (nb == CENTER ? slot : take);
-            f[nb] = takef ? a.arg[e2] : 0.0;
         }
+        double fc = (has_arg && slot) ? a.arg[c + C * lane] : 0.0;
         const bool act0 = (o9[CENTER] >> lane) & 1;
         double xv[2], dq[2];
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
             xv[d] = g.xv[d].x[I[d]];
             dq[d] = g.xv[d].dx[I[d]] / 2.0;
+        }
+        // the random numbers of a new particle depend on (cell, slot, dim)
+        // only, not on its quadrant: drawn once, in the free slots
+        double r2[2] = {0.0, 0.0};
+        if (slot && !act0) {
+            r2[0] = uniform53(a.seed, (uint64_t)c, lane, 0, 2, a.counter);
+            r2[1] = uniform53(a.seed, (uint64_t)c, lane, 1, 2, a.counter);
         }
         // plan: the quadrants in order, each topped up from the lowest free
         // slots until it holds min_q particles; the first full quadrant ends
@@ -149,23 +156,22 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
                 act = true;
                 if constexpr (SPLIT) {
                     const double q0 = 0.5 * (double)(qd & 1), q1 = 0.5 * (double)((qd >> 1) & 1);
-                    q[CENTER][0] = (float)(q0 + 0.5 * fma(0.95, uniform53(a.seed, (uint64_t)c, lane,
-                                                                         0, 2, a.counter), 0.05));
-                    q[CENTER][1] = (float)(q1 + 0.5 * fma(0.95, uniform53(a.seed, (uint64_t)c, lane,
-                                                                         1, 2, a.counter), 0.05));
+                    q[CENTER][0] = (float)(q0 + 0.5 * fma(0.95, r2[0], 0.05));
+                    q[CENTER][1] = (float)(q1 + 0.5 * fma(0.95, r2[1], 0.05));
                 } else {
 #pragma unroll
                     for (int d = 0; d < 2; ++d) {
                         const double v = xv[d] + dq[d] * (double)((qd >> d) & 1);
-                        const double r = uniform53(a.seed, (uint64_t)c, lane, d, 2, a.counter);
-                        q[CENTER][d] = v + dq[d] * fma(0.95, r, 0.05);
+                        q[CENTER][d] = v + dq[d] * fma(0.95, r2[d], 0.05);
                     }
                 }
             }
         }
-        // the searches, in injection (= ascending slot) order
-        double fc = f[CENTER];
-        for (uint64_t bits = injm; bits; bits &= bits - 1) {
+        // the searches, in injection (= ascending slot) order; lane j of
+        // `src` records injection j's nearest particle (order nb*64 + slot)
+        int src = 0x7fffffff;
+        int j = 0;
+        for (uint64_t bits = injm; bits; bits &= bits - 1, ++j) {
             const int ip = __builtin_ctzll(bits);
             P pn[2];
             if constexpr (SPLIT) {
@@ -186,38 +192,52 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
                 s9[nb] = __builtin_inf();
                 const bool av = nb == CENTER ? cand_c : (bool)((o9[nb] >> lane) & 1);
                 if (!av) continue;
-                double s;
+                double sq;
                 if constexpr (SPLIT) {
                     const double r0 =
                         ((double)(nb % 3 - 1) + (double)q[nb][0] - (double)pn[0]) * a.h[0];
                     const double r1 =
                         ((double)(nb / 3 - 1) + (double)q[nb][1] - (double)pn[1]) * a.h[1];
-                    s = r0 * r0 + r1 * r1;
+                    sq = r0 * r0 + r1 * r1;
                 } else {
                     if (q[nb][0] != q[nb][0] || q[nb][1] != q[nb][1]) continue;  // 2D NaN skip
                     const double du = q[nb][0] - pn[0], dv = q[nb][1] - pn[1];
-                    s = du * du + dv * dv;
+                    sq = du * du + dv * dv;
                 }
-                s9[nb] = s;
-                if (s < best) {
-                    best = s;
+                s9[nb] = sq;
+                if (sq < best) {
+                    best = sq;
                     border = nb * 64 + lane;
                 }
             }
             wave_argmin(best, border);
             resolve_ties(s9, best, border, lane);
-            if (border == 0x7fffffff) {
-                ++n_orph;  // no neighbour at all: the field keeps its slot's value
-            } else if (has_arg) {
-                const int nbw = border >> 6, lw = border & 63;  // uniform
-                double src = fc;
+            if (lane == j) src = border;
+        }
+        const int ninj = j;
+        if (has_arg && ninj) {
+            // values of the sources in neighbour cells: one load per
+            // injection, all in flight together
+            double nv = 0.0;
+            if (lane < ninj && src != 0x7fffffff && (src >> 6) != CENTER) {
+                const int nbw = src >> 6;
+                int64_t cs = cn[0];
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    if (nb != CENTER && nb == nbw) src = f[nb];
-                const double v = readlane_d(src, lw);
+                for (int nb = 1; nb < NB; ++nb) cs = nb == nbw ? cn[nb] : cs;
+                nv = a.arg[cs + C * (src & 63)];
+            }
+            // the copies in injection order: a source in this cell is read
+            // from the current values (an earlier injection's copy included)
+            j = 0;
+            for (uint64_t bits = injm; bits; bits &= bits - 1, ++j) {
+                const int ip = __builtin_ctzll(bits);
+                const int sj = __builtin_amdgcn_readlane(src, j);
+                if (sj == 0x7fffffff) continue;  // orphan: the slot keeps its value
+                const double v = (sj >> 6) == CENTER ? readlane_d(fc, sj & 63) : readlane_d(nv, j);
                 if (lane == ip) fc = v;
             }
         }
+        n_orph += __popcll(__ballot(lane < ninj && src == 0x7fffffff));
         n_inj += __popcll(injm);
         if ((injm >> lane) & 1) {
             const int64_t e = c + C * lane;
