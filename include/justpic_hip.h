@@ -103,11 +103,15 @@ void* jp_ctx_stream(jp_ctx* ctx);
  *                       set of axes (pointers and lengths) until any jp_free,
  *                       1 on every call (for axes whose coordinates the
  *                       caller changes in place or frees outside jp_free).
+ *   JP_OPT_MOVE_PHASES  the 2D move's colour phases: 0 (default) all nine in
+ *                       one launch over tiles of cells, 1 one launch per
+ *                       phase (the form the 3D move always takes).
  * No reference counterpart (the KernelAbstractions launch sizes of the
  * reference are chosen by its backend). */
 #define JP_OPT_SLOT_SPAN 1
 #define JP_OPT_INJECT_WAVES 2
 #define JP_OPT_GEOMETRY_RECHECK 3
+#define JP_OPT_MOVE_PHASES 4
 int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value);
 int jp_sync(jp_ctx* ctx);
 int jp_malloc(jp_ctx* ctx, void** dptr, size_t bytes);

@@ -358,6 +358,10 @@ int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value) {
             return fail(ctx, JP_ERR_ARG, "geometry recheck must be 0 or 1%s");
         ctx->opt_geom_recheck = (int)value;
         return JP_OK;
+    case JP_OPT_MOVE_PHASES:
+        if (value < 0 || value > 1) return fail(ctx, JP_ERR_ARG, "move phases must be 0 or 1%s");
+        ctx->opt_move_phases = (int)value;
+        return JP_OK;
     default:
         return fail(ctx, JP_ERR_ARG, "unknown option%s %lld", "", option);
     }

@@ -479,6 +479,228 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     stat_add(st, ST_DROPPED, n_drop);
 }
 
+// ------------------------------------------------------------ 2'. 2D phases in tiles
+// All nine colour phases of the 2D move in one launch.  A workgroup owns a
+// tile of cells and replays the phases on the occupancy, pending, arrival and
+// touched words of the tile plus a halo, all in LDS, with a barrier between
+// phases; it keeps the results of its own cells.
+//
+// The halo is the dependency cone of the phases.  The processing of cell A in
+// phase p reads and writes the words of A's 3x3 neighbourhood, so it depends
+// on the processings, in earlier phases, of the cells B within 2 of A (their
+// neighbourhoods overlap A's), and on theirs in turn.  A step from A to B
+// moves, per dimension, by a multiple of 3 plus the difference of their
+// colours, within [-2, 2]; taking the phases down from p, the offsets that can
+// be reached lie in lo_p..hi_p with, for phase p = 3 c0 + c1 (c0 = I0 mod 3,
+// the outer digit of the phase, c1 = I1 mod 3):
+//   dimension 0: -c0 .. 2 c0,  dimension 1: -p .. 2 p.
+// A processing is exact here when A + [lo_p - 1, hi_p + 1] lies inside the
+// region (or leaves the grid there), and a processing that changes an exact
+// one is itself exact, so only those run.  The final words of an owned cell
+// come from the processings of its 3x3 neighbourhood, so the region is the
+// tile widened by the largest cone plus 2: 4 cells below and 6 above in
+// dimension 0, 10 below and 18 above in dimension 1.
+//
+// Tiles overlap, so a halo processing also runs in the tile that owns it;
+// both compute the same slots from the same exact inputs.  Every processing
+// here stores its origins (equal values where tiles overlap), since a later
+// phase of this workgroup reads the origin of an arrival that moves on;
+// counts and final words come from owned cells only.
+//
+// A pending slot that holds an arrival of this call is always re-slotted in
+// its own cell (an arrival that fails the strict test of its new cell, or a
+// particle re-slotted higher in its own cell), so the destination code of
+// such a slot is the centre and the code words are read-only here.
+constexpr int kTileHalo0Lo = 4, kTileHalo0Hi = 6, kTileHalo1Lo = 10, kTileHalo1Hi = 18;
+
+__device__ inline int cone_lo(int p, int d) { return d == 0 ? -(p / 3) : -p; }
+__device__ inline int cone_hi(int p, int d) { return d == 0 ? 2 * (p / 3) : 2 * p; }
+
+template <int TX, int TY, int BS, bool STATS>
+__global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
+                                                  const int* __restrict__ flag, jp_stats* st) {
+    constexpr int RX = TX + kTileHalo0Lo + kTileHalo0Hi;
+    constexpr int RY = TY + kTileHalo1Lo + kTileHalo1Hi;
+    constexpr int R = RX * RY;
+    static_assert(((RX + 2) / 3) * ((RY + 2) / 3) <= BS, "one cell per thread and phase");
+    __shared__ uint64_t s_occ[R], s_mov[R], s_arr[R], s_tch[R];
+    // the code bytes of the cell each thread is processing, word j of thread t
+    // at s_code[j][t] (a lane's byte reads hit banks 8 bytes apart)
+    __shared__ uint64_t s_code[8][BS];
+    if (*flag) return;  // far jump or rejected grid: nothing moves here
+    const int n0 = (int)g.ni[0], n1 = (int)g.ni[1];
+    const int ntx = (n0 + TX - 1) / TX;
+    // neighbouring tiles read each other's halo words: consecutive tiles of
+    // a row on one XCD
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int tx0 = (tile % ntx) * TX, ty0 = (tile / ntx) * TY;
+    const int tx1 = (tx0 + TX < n0 ? tx0 + TX : n0) - 1;
+    const int ty1 = (ty0 + TY < n1 ? ty0 + TY : n1) - 1;
+    const int gx0 = tx0 - kTileHalo0Lo > 0 ? tx0 - kTileHalo0Lo : 0;
+    const int gx1 = tx1 + kTileHalo0Hi < n0 - 1 ? tx1 + kTileHalo0Hi : n0 - 1;
+    const int gy0 = ty0 - kTileHalo1Lo > 0 ? ty0 - kTileHalo1Lo : 0;
+    const int gy1 = ty1 + kTileHalo1Hi < n1 - 1 ? ty1 + kTileHalo1Hi : n1 - 1;
+    const int rw = gx1 - gx0 + 1, rh = gy1 - gy0 + 1;
+    for (int i = threadIdx.x; i < rw * rh; i += BS) {
+        const int x = i % rw, y = i / rw;
+        const uint32_t c = (uint32_t)(gx0 + x) + (uint32_t)n0 * (uint32_t)(gy0 + y);
+        const int r = x + RX * y;
+        s_occ[r] = ms.occ0[c];
+        s_mov[r] = ms.mov[c];
+        s_arr[r] = 0;
+        s_tch[r] = 0;
+    }
+    __syncthreads();
+    const int K = g.K;
+    const uint32_t C32 = (uint32_t)g.C;
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    const int nw = (K + 7) >> 3;
+    const uint8_t* __restrict__ sc = reinterpret_cast<const uint8_t*>(&s_code[0][threadIdx.x]);
+    int n_moved = 0, n_del = 0, n_drop = 0;
+    // the colour-p cells whose processing is exact in the region: x = vx +
+    // 3i (i < cx), y = vy + 3j (j < cy)
+    auto box = [&](int p, int& vx, int& vy, int& cx, int& cy) {
+        const int ax = gx0 == 0 ? 0 : gx0 - cone_lo(p, 0) + 1;
+        const int bx = gx1 == n0 - 1 ? n0 - 1 : gx1 - cone_hi(p, 0) - 1;
+        const int ay = gy0 == 0 ? 0 : gy0 - cone_lo(p, 1) + 1;
+        const int by = gy1 == n1 - 1 ? n1 - 1 : gy1 - cone_hi(p, 1) - 1;
+        vx = ax + (p / 3 - ax % 3 + 3) % 3;
+        vy = ay + (p % 3 - ay % 3 + 3) % 3;
+        cx = vx <= bx ? (bx - vx) / 3 + 1 : 0;
+        cy = vy <= by ? (by - vy) / 3 + 1 : 0;
+    };
+    auto load_codes = [&](uint32_t S32, uint64_t (&w)[8]) {
+        const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S32;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) w[j] = j < nw ? cp[(int64_t)j * C32] : 0ull;
+    };
+    auto cell_at = [&](int vx, int vy, int cx, int t) {
+        return (uint32_t)(vx + 3 * (t % cx)) + (uint32_t)n0 * (uint32_t)(vy + 3 * (t / cx));
+    };
+    // the code words of each thread's first cell of a phase are loaded one
+    // phase ahead, so that they are in flight during the phase before
+    uint64_t nxt[8];
+    {
+        int vx, vy, cx, cy;
+        box(0, vx, vy, cx, cy);
+        if ((int)threadIdx.x < cx * cy) load_codes(cell_at(vx, vy, cx, threadIdx.x), nxt);
+    }
+    for (int p = 0; p < kNB<2>; ++p) {
+        const int col0 = p / 3, col1 = p % 3;
+        int vx, vy, cx, cy;
+        box(p, vx, vy, cx, cy);
+        if ((int)threadIdx.x < cx * cy) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < nw) s_code[j][threadIdx.x] = nxt[j];
+        }
+        if (p + 1 < kNB<2>) {
+            int wx, wy, dx_, dy_;
+            box(p + 1, wx, wy, dx_, dy_);
+            if ((int)threadIdx.x < dx_ * dy_) load_codes(cell_at(wx, wy, dx_, threadIdx.x), nxt);
+        }
+        const int t = threadIdx.x;  // at most one cell of the colour per thread
+        uint64_t pend = 0;
+        int x = 0, y = 0, r = 0;
+        if (t < cx * cy) {
+            x = vx + 3 * (t % cx);
+            y = vy + 3 * (t / cx);
+            r = (x - gx0) + RX * (y - gy0);
+            pend = s_mov[r];
+        }
+        if (pend) {
+            const bool own = tx0 <= x && x <= tx1 && ty0 <= y && y <= ty1;
+            const uint32_t S32 = (uint32_t)x + (uint32_t)n0 * (uint32_t)y;
+            // neighbours inside the grid (bit n)
+            uint32_t ingm = 0;
+#pragma unroll
+            for (int n = 0; n < kNB<2>; ++n) {
+                const int jx = x + n % 3 - 1, jy = y + n / 3 - 1;
+                ingm |= (uint32_t)(jx >= 0 && jx < n0 && jy >= 0 && jy < n1) << n;
+            }
+            uint64_t occC = s_occ[r], arrS = s_arr[r], tchS = s_tch[r];
+            uint64_t free_sp = full;  // slots >= starting_point
+            while (pend) {
+                const int ip = __builtin_ctzll(pend);
+                pend &= pend - 1;
+                uint32_t cd = sc[(ip >> 3) * (BS * 8) + (ip & 7)];
+                const uint64_t bit = 1ull << ip;
+                occC &= ~bit;  // empty_particle! on the source slot
+                uint32_t origin = S32 + C32 * (uint32_t)ip;
+                if (arrS & bit) {  // rare: an arrival of this call is re-slotted
+                    origin = ms.org[org_at<2>(S32, (uint32_t)ip, C32, (uint32_t)K)];
+                    // waited for here, on the rare path: a wait at the loop
+                    // head would also wait for the loop's origin stores (one
+                    // counter for vector loads and stores)
+                    __builtin_amdgcn_s_waitcnt(kVmcnt0);
+                    arrS &= ~bit;
+                    cd = kCenter<2>;
+                }
+                if (cd == kDel) {
+                    if (STATS) n_del += own;
+                    continue;
+                }
+                const int n = cd & 0x7F;
+                const int q3 = (n * 11) >> 5;  // n / 3 for n < 9
+                const int dx = n - 3 * q3 - 1, dy = q3 - 1;
+                int rn = r + dx + RX * dy;  // inside the region when in the grid
+                rn = rn < 0 ? 0 : (rn > R - 1 ? R - 1 : rn);
+                const uint64_t ol = s_occ[rn];
+                const bool centre = n == kCenter<2>;
+                // outside the grid: never free
+                const uint64_t od = centre ? occC : (((ingm >> n) & 1) ? ol : ~0ull);
+                const uint64_t fr = ~od & free_sp;
+                if (!fr) {  // find_free_memory found nothing: dropped
+                    if (STATS) n_drop += own;
+                    continue;
+                }
+                const int f = __builtin_ctzll(fr);
+                const uint64_t fb = 1ull << f;
+                free_sp = full & (~0ull << f);  // starting_point = free_idx
+                if (STATS) n_moved += own;
+                const uint32_t dcn = S32 + (uint32_t)(dx + n0 * dy);
+                ms.org[org_at<2>(dcn, (uint32_t)f, C32, (uint32_t)K)] = origin;
+                if (centre) {
+                    occC |= fb;
+                    arrS |= fb;
+                    tchS |= fb;
+                    if (f > ip) pend |= fb;  // the slot loop reaches it again
+                } else {
+                    s_occ[rn] = od | fb;
+                    atomicOr(reinterpret_cast<unsigned long long*>(&s_arr[rn]),
+                             (unsigned long long)fb);
+                    atomicOr(reinterpret_cast<unsigned long long*>(&s_tch[rn]),
+                             (unsigned long long)fb);
+                    // an arrival that fails the strict test of its new cell is
+                    // re-slotted there when that cell's phase comes later
+                    if ((cd & kGap) && ((col0 + dx + 3) % 3) * 3 + (col1 + dy + 3) % 3 > p)
+                        atomicOr(reinterpret_cast<unsigned long long*>(&s_mov[rn]),
+                                 (unsigned long long)fb);
+                }
+            }
+            s_occ[r] = occC;
+            s_arr[r] = arrS;
+            s_tch[r] = tchS;
+            s_mov[r] = 0;
+        }
+        __syncthreads();
+    }
+    const int ow = tx1 - tx0 + 1, oh = ty1 - ty0 + 1;
+    for (int i = threadIdx.x; i < ow * oh; i += BS) {
+        const int x = i % ow, y = i / ow;
+        const uint32_t c = (uint32_t)(tx0 + x) + (uint32_t)n0 * (uint32_t)(ty0 + y);
+        const int r = (tx0 - gx0) + x + RX * ((ty0 - gy0) + y);
+        ms.occ[c] = s_occ[r];
+        ms.arr[c] = s_arr[r];
+        ms.tch[c] = s_tch[r];
+    }
+    if (STATS) {
+        stat_add(st, ST_MOVED, n_moved);
+        stat_add(st, ST_DELETED, n_del);
+        stat_add(st, ST_DROPPED, n_drop);
+    }
+}
+
 // ------------------------------------------------------------ 3. scatter
 // T: coordinate type (double: FP64 drop-in, float: split offsets)
 template <int D, typename T>
@@ -904,7 +1126,22 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     int64_t total = 1;
     for (int d = 0; d < D; ++d) total *= (g.ni[d] + 2) / 3;
     constexpr int BS = D == 2 ? 128 : 64;  // measured: 256 and 64 (2D), 128 (3D) slower
-    const int nphase = kNB<D>;
+    const int nphase = (D == 2 && !ctx->opt_move_phases) ? 0 : kNB<D>;
+    if (D == 2 && !ctx->opt_move_phases) {
+        constexpr int TX = 32, TY = 16;
+        const int64_t tiles = ((g.ni[0] + TX - 1) / TX) * ((g.ni[1] + TY - 1) / TY);
+        if (tiles >= ((int64_t)1 << 31) || g.ni[0] >= ((int64_t)1 << 30) ||
+            g.ni[1] >= ((int64_t)1 << 30))
+            return fail(ctx, JP_ERR_ARG, "grid too large%s");
+        Grid<2> g2 = to_grid<2>(gc);
+        if (st)
+            k_mv_tiles2<TX, TY, 256, true><<<(unsigned)tiles, 256, 0, ctx->stream>>>(g2, ms, flag,
+                                                                                   st);
+        else
+            k_mv_tiles2<TX, TY, 256, false><<<(unsigned)tiles, 256, 0, ctx->stream>>>(g2, ms,
+                                                                                    flag, st);
+        JP_LAUNCHED(ctx);
+    }
     for (int ph = 0; ph < nphase; ++ph) {
         Off3 o{{0, 0, 0}};
         int r = ph;
