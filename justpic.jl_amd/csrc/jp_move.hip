@@ -21,11 +21,13 @@
 //      into their final slot, vacated slots get NaN and mask false.
 //
 // A particle that jumps more than one cell breaks the colour independence;
-// classify raises a flag, the phases and the scatter return immediately, and
-// a single-thread kernel, launched last, runs the reference loop for every
-// phase in order and clears the flag.  The split classify raises another bit
-// of the same flag when the grid's geometry was rejected: then nothing moves
-// and the last kernel only clears the flag.
+// classify raises a flag (and records the largest jump), the phases and the
+// scatter return immediately, and a one-workgroup kernel, launched last, runs
+// the reference loop on the particle data for every phase, cells in levels
+// that keep the reference's order wherever two cells interact, and clears
+// the flag.  The split classify raises another bit of the same flag when the
+// grid's geometry was rejected: then nothing moves and the last kernel only
+// clears the flag.
 #include "jp_ctx.hpp"
 
 #include <type_traits>
@@ -67,25 +69,6 @@ struct MoveScratch {
 template <int D>
 __device__ inline uint32_t org_at(uint32_t c, uint32_t k, uint32_t C, uint32_t K) {
     return D == 2 ? c + C * k : c * K + k;
-}
-
-template <int D>
-__device__ inline bool in_cell_at(const Grid<D>& g, const int64_t* I, const double* p) {
-    bool b = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        double x = g.xv[d].x[I[d]], h = g.xv[d].dx[I[d]];
-        b = b & (x < p[d] && p[d] < x + h);  // isincell, src/Particles/utils.jl:15
-    }
-    return b;
-}
-
-template <int D>
-__device__ inline bool in_domain(const Grid<D>& g, const double* p) {
-#pragma unroll
-    for (int d = 0; d < D; ++d)  // indomain, src/Particles/move_safe.jl:199-206
-        if (!(g.xv[d].x[0] < p[d] && p[d] < g.xv[d].x[g.xv[d].n - 1])) return false;
-    return true;
 }
 
 template <int D>
@@ -243,6 +226,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
             } else {
                 int n = 0, mult = 1;
                 bool far = false, inJ = true;
+                int64_t jump = 0;  // cells jumped (largest over the dimensions)
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     int r;
@@ -252,6 +236,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     else {
                         const int64_t t = find_cell(p[d], g.xv[d], I[d]) - I[d];
                         r = t < -1 ? -2 : (t > 1 ? 2 : (int)t);
+                        jump = t < -jump ? -t : (t > jump ? t : jump);
                     }
                     far = far || r < -1 || r > 1;
                     n += (r + 1) * mult;
@@ -262,7 +247,8 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                 }
                 if (far) {
                     atomicOr(flag, kFlagFar);
-                    cd = kDel;  // unused: the serial kernel takes over
+                    atomicMax(flag + 1, (int)(jump < (1 << 30) ? jump : (1 << 30)));
+                    cd = kDel;  // unused: the far-jump kernel takes over
                 } else {
                     cd = (uint8_t)n;
                     if (n != kCenter<D> && !inJ) cd |= kGap;
@@ -307,7 +293,7 @@ struct alignas(16) NbEntry {
 };
 
 // Colour cell t of phase `phase` (offsets `off`): the reference slot loop of
-// that cell.  `stop`: the serial fallback runs instead (checked after the
+// that cell.  `stop`: the far-jump kernel runs instead (checked after the
 // cell's loads are issued).
 template <int D, int BS>
 __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const Off3& off,
@@ -367,7 +353,7 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
         arrN[n] = side ? ms.arr[nbc[n]] : 0ull;
         tchN[n] = side ? ms.tch[nbc[n]] : 0ull;
     }
-    if (stop || !pend) return;  // serial fallback, or nothing leaves or re-slots here
+    if (stop || !pend) return;  // far-jump path, or nothing leaves or re-slots here
 #pragma unroll
     for (int n = 0; n < kNB<D>; ++n) s_nb[n][tid] = NbEntry{was[n], (uint32_t)nbc[n], 0u};
 #pragma unroll
@@ -786,99 +772,86 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
     }
 }
 
-// ------------------------------------------------------------ serial paths
+// ------------------------------------------------------------ far jumps
 // move_kernel! (src/Particles/move_safe.jl:72-125) for one cell, directly on
-// the data (used when some particle jumps more than one cell).
-template <int D>
-__device__ void move_cell_serial(const Grid<D>& g, const TPtr3<D, double>& coords, uint8_t* index,
-                                 int nargs, const ArgPtrs& args, const int64_t* I,
-                                 int64_t (&cnt)[4]) {
+// the data (the far-jump path).  The cell's occupancy and, per mover, the
+// destination's are read as one round of independent byte loads, and the
+// slot loop runs on those words: the slots are visited in order (a particle
+// re-slotted higher in its own cell is visited again, as in the reference),
+// the destination slot is the first free one at or after starting_point.
+// FP64 positions: isincell / indomain, then the bisection seeded at the
+// cell; split offsets: the destination is the cell + floor(u) and the offset
+// is renormalised to it.
+template <int D, typename T>
+__device__ void move_cell_far(const Grid<D>& g, const TPtr3<D, T>& coords, uint8_t* index,
+                              int nargs, const ArgPtrs& args, const int64_t* I,
+                              int64_t (&cnt)[4]) {
+    constexpr bool SPLIT = std::is_same<T, float>::value;
     const int64_t C = g.C;
     const int K = g.K;
     const int64_t c = lin<D>(g.ni, I);
+    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
+    double xlo[D], hx[D], dlo[D], dhi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if constexpr (!SPLIT) {
+            const Axis& A = g.xv[d];
+            xlo[d] = A.x[I[d]];
+            hx[d] = A.dx[I[d]];
+            dlo[d] = A.x[0];
+            dhi[d] = A.x[A.n - 1];
+        } else {
+            xlo[d] = hx[d] = dlo[d] = dhi[d] = 0.0;
+        }
+    }
+    uint64_t own = active_slots(index, c, C, K);
+    uint64_t todo = own;
     int sp = 0;
-    for (int ip = 0; ip < K; ++ip) {
+    while (todo) {
+        const int ip = __builtin_ctzll(todo);
+        todo &= todo - 1;
         const int64_t e = c + C * ip;
-        if (!index[e]) continue;
-        double p[D];
+        T p[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-        if (in_cell_at<D>(g, I, p)) continue;
-        if (!in_domain<D>(g, p)) {
-            index[e] = 0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
-            for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-            cnt[1]++;
-            continue;
-        }
-        int64_t nc[D];
-        bool far = false;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            nc[d] = find_cell(p[d], g.xv[d], I[d]);
-            far = far || (nc[d] - I[d] > 1 || I[d] - nc[d] > 1);
-        }
-        if (far) cnt[3]++;
-        double cached[JP_MAX_ARGS];
-        for (int a = 0; a < nargs; ++a) cached[a] = args.p[a][e];
-        index[e] = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = dnan();
-        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-        const int64_t cn = lin<D>(g.ni, nc);
-        int fi = -1;
-        for (int i = sp; i < K; ++i)
-            if (!index[cn + C * i]) {
-                fi = i;
-                break;
-            }
-        if (fi < 0) {
-            cnt[2]++;
-            continue;
-        }
-        sp = fi;
-        const int64_t f = cn + C * fi;
-        index[f] = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][f] = p[d];
-        for (int a = 0; a < nargs; ++a) args.p[a][f] = cached[a];
-        cnt[0]++;
-    }
-}
-
-// The same loop for split positions: the cell offset is floor(u); the
-// renormalised offset u - floor(u) moves with the particle.
-template <int D>
-__device__ void move_cell_serial(const Grid<D>& g, const TPtr3<D, float>& off, uint8_t* index,
-                                 int nargs, const ArgPtrs& args, const int64_t* I,
-                                 int64_t (&cnt)[4]) {
-    const int64_t C = g.C;
-    const int K = g.K;
-    const int64_t c = lin<D>(g.ni, I);
-    int sp = 0;
-    for (int ip = 0; ip < K; ++ip) {
-        const int64_t e = c + C * ip;
-        if (!index[e]) continue;
-        float u[D];
         int64_t J[D];
-        bool stay = true, out = false, far = false;
+        bool out = false, far = false;
+        if constexpr (SPLIT) {
+            bool stay = true;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            u[d] = off.p[d][e];
-            const bool fin = fabsf(u[d]) < 1.0e6f;  // false for NaN/Inf
-            const int s = fin ? (int)floorf(u[d]) : 0;
-            out = out || !fin;
-            J[d] = I[d] + s;
-            stay = stay && s == 0;
-            far = far || s > 1 || s < -1;
-            out = out || J[d] < 0 || J[d] >= g.ni[d];
+            for (int d = 0; d < D; ++d) {
+                const bool fin = fabsf(p[d]) < 1.0e6f;  // false for NaN/Inf
+                const int sd = fin ? (int)floorf(p[d]) : 0;
+                out = out || !fin;
+                J[d] = I[d] + sd;
+                stay = stay && sd == 0;
+                far = far || sd > 1 || sd < -1;
+                out = out || J[d] < 0 || J[d] >= g.ni[d];
+            }
+            if (stay && !out) continue;
+        } else {
+            bool inc = true, indom = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                inc = inc & (xlo[d] < p[d] && p[d] < xlo[d] + hx[d]);  // isincell
+                indom = indom & (dlo[d] < p[d] && p[d] < dhi[d]);      // indomain
+            }
+            if (inc) continue;
+            out = !indom;
+            if (!out) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    J[d] = find_cell(p[d], g.xv[d], I[d]);
+                    far = far || J[d] - I[d] > 1 || I[d] - J[d] > 1;
+                }
+            }
         }
-        if (stay && !out) continue;
-        if (out) {
+        const uint64_t bit = 1ull << ip;
+        own &= ~bit;
+        if (out) {  // leaves the domain: emptied
             index[e] = 0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) off.p[d][e] = __builtin_nanf("");
+            for (int d = 0; d < D; ++d) coords.p[d][e] = (T)dnan();
             for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
             cnt[1]++;
             continue;
@@ -888,68 +861,114 @@ __device__ void move_cell_serial(const Grid<D>& g, const TPtr3<D, float>& off, u
         for (int a = 0; a < nargs; ++a) cached[a] = args.p[a][e];
         index[e] = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) off.p[d][e] = __builtin_nanf("");
+        for (int d = 0; d < D; ++d) coords.p[d][e] = (T)dnan();
         for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
         const int64_t cn = lin<D>(g.ni, J);
-        int fi = -1;
-        for (int i = sp; i < K; ++i)
-            if (!index[cn + C * i]) {
-                fi = i;
-                break;
-            }
-        if (fi < 0) {
+        const uint64_t dm = cn == c ? own : active_slots(index, cn, C, K);
+        const uint64_t fr = ~dm & full & (~0ull << sp);
+        if (!fr) {  // find_free_memory found nothing: dropped
             cnt[2]++;
             continue;
         }
-        sp = fi;
+        const int fi = __builtin_ctzll(fr);
+        sp = fi;  // starting_point = free_idx
         const int64_t f = cn + C * fi;
         index[f] = 1;
 #pragma unroll
-        for (int d = 0; d < D; ++d) off.p[d][f] = (float)((double)u[d] - (double)(J[d] - I[d]));
+        for (int d = 0; d < D; ++d) {
+            if constexpr (SPLIT)
+                coords.p[d][f] = (float)((double)p[d] - (double)(J[d] - I[d]));
+            else
+                coords.p[d][f] = p[d];
+        }
         for (int a = 0; a < nargs; ++a) args.p[a][f] = cached[a];
         cnt[0]++;
+        if (cn == c) {  // re-slotted in its own cell
+            own |= 1ull << fi;
+            if (fi > ip) todo |= 1ull << fi;
+        }
     }
 }
 
+// The reference loop of every cell, phase after phase, when some particle
+// jumps more than one cell (flag[0] has kFlagFar, flag[1] holds the largest
+// jump R in cells).  One workgroup: a cell's processing touches only cells
+// within R of it, so two cells of one colour phase (3t + o) interact only
+// when their colour coordinates t differ by at most q = floor(2R/3) in every
+// dimension.  Within a phase the reference visits the cells in linear order
+// of t; here they go in levels L(t) = sum_d m_d t_d with m_0 = 1 and
+// m_d = 1 + sum_{e<d} m_e q_e (q_e = q capped at the colour count - 1): any
+// two interacting cells have their linear order as level order, and cells of
+// one level do not interact, so each level runs in parallel between
+// barriers and the result is the reference's.  With a small R that is
+// 3^D * (about (1 + m_1 + m_2) * the colour count per dimension) levels; a
+// jump of the grid's size makes every level one cell (the serial order).
+// Also clears the flag words after a rejected split grid.
 template <int D, typename T>
-__global__ void k_mv_serial(Grid<D> g, TPtr3<D, T> coords, uint8_t* index, int nargs,
-                            ArgPtrs args, int* __restrict__ flag, jp_stats* st) {
-    if (!*flag || blockIdx.x != 0 || threadIdx.x != 0) return;
-    if (*flag & kFlagRejected) {  // split grid rejected: nothing moves
-        *flag = 0;
-        return;
-    }
-    int64_t ncol[D];
-    int64_t total = 1;
-    for (int d = 0; d < D; ++d) {
-        ncol[d] = (g.ni[d] + 2) / 3;
-        total *= ncol[d];
-    }
-    int64_t cnt[4] = {0, 0, 0, 0};
-    const int nph = kNB<D>;
-    for (int ph = 0; ph < nph; ++ph) {
-        int o[D];
-        int r = ph;
-        for (int d = D - 1; d >= 0; --d) {
-            o[d] = r % 3;
-            r /= 3;
+__global__ void __launch_bounds__(1024) k_mv_far(Grid<D> g, TPtr3<D, T> coords, uint8_t* index,
+                                                 int nargs, ArgPtrs args, int* __restrict__ flag,
+                                                 jp_stats* st) {
+    const int fl = *flag;
+    if (!fl) return;
+    const int R = flag[1];
+    __syncthreads();  // every thread has read the words before they are cleared
+    if (!(fl & kFlagRejected)) {
+        int64_t ncol[D], m[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) ncol[d] = (g.ni[d] + 2) / 3;
+        const int64_t q = (2 * (int64_t)R) / 3;
+        int64_t lmax = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m[d] = 1;
+            for (int e = 0; e < d; ++e) m[d] += m[e] * (q < ncol[e] - 1 ? q : ncol[e] - 1);
+            lmax += m[d] * (ncol[d] - 1);
         }
-        for (int64_t t = 0; t < total; ++t) {
-            int64_t q = t, I[D];
-            bool ok = true;
-            for (int d = 0; d < D; ++d) {
-                I[d] = 3 * (q % ncol[d]) + o[d];
-                q /= ncol[d];
-                ok = ok && I[d] < g.ni[d];
+        // the (t_1, ..., t_{D-1}) combinations, one per thread and round
+        int64_t outer = 1;
+#pragma unroll
+        for (int d = 1; d < D; ++d) outer *= ncol[d];
+        int64_t cnt[4] = {0, 0, 0, 0};
+        for (int ph = 0; ph < kNB<D>; ++ph) {
+            int o[D];
+            int rr = ph;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) {
+                o[d] = rr % 3;
+                rr /= 3;
             }
-            if (ok) move_cell_serial<D>(g, coords, index, nargs, args, I, cnt);
+            for (int64_t lv = 0; lv <= lmax; ++lv) {
+                for (int64_t i = threadIdx.x; i < outer; i += blockDim.x) {
+                    int64_t t[D], rest = i, l0 = lv;
+#pragma unroll
+                    for (int d = 1; d < D; ++d) {
+                        t[d] = rest % ncol[d];
+                        rest /= ncol[d];
+                        l0 -= m[d] * t[d];
+                    }
+                    t[0] = l0;
+                    if (t[0] < 0 || t[0] >= ncol[0]) continue;
+                    int64_t I[D];
+                    bool ok = true;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        I[d] = 3 * t[d] + o[d];
+                        ok = ok && I[d] < g.ni[d];
+                    }
+                    if (ok) move_cell_far<D, T>(g, coords, index, nargs, args, I, cnt);
+                }
+                __syncthreads();
+            }
         }
+        stat_add(st, ST_MOVED, cnt[0]);
+        stat_add(st, ST_DELETED, cnt[1]);
+        stat_add(st, ST_DROPPED, cnt[2]);
+        stat_add(st, ST_FAR, cnt[3]);
     }
-    stat_add(st, ST_MOVED, cnt[0]);
-    stat_add(st, ST_DELETED, cnt[1]);
-    stat_add(st, ST_DROPPED, cnt[2]);
-    stat_add(st, ST_FAR, cnt[3]);
-    *flag = 0;  // ready for the next move on this context
+    if (threadIdx.x == 0) {  // ready for the next move on this context
+        flag[0] = 0;
+        flag[1] = 0;
+    }
 }
 
 // ------------------------------------------------------------ split classify
@@ -1001,7 +1020,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
             if (!((occ >> k) & 1)) continue;
             int s[D];
             bool stay = true, out = false, far = false;
-            int n = 0, mult = 1;
+            int n = 0, mult = 1, jump = 0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const bool fin = fabsf(ub[j][d]) < 1.0e6f;  // false for NaN/Inf
@@ -1011,6 +1030,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 out = out || Jd < 0 || Jd >= g.ni[d];
                 stay = stay && s[d] == 0;
                 far = far || s[d] > 1 || s[d] < -1;
+                jump = s[d] < -jump ? -s[d] : (s[d] > jump ? s[d] : jump);
                 n += (s[d] + 1) * mult;
                 mult *= 3;
             }
@@ -1021,7 +1041,8 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 cd = kDel;
             } else if (far) {
                 atomicOr(flag, kFlagFar);
-                cd = kDel;  // unused: the serial kernel takes over
+                atomicMax(flag + 1, jump);
+                cd = kDel;  // unused: the far-jump kernel takes over
             } else {
                 cd = (uint8_t)n;
 #pragma unroll
@@ -1058,8 +1079,8 @@ TPtr3<D, T> tptr3(T* const* c) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// classify -> 3^D colour phases -> scatter -> serial fallback (after a far
-// jump the phases and the scatter return at once, and the serial kernel moves
+// classify -> 3^D colour phases -> scatter -> far-jump kernel (after a far
+// jump the phases and the scatter return at once, and the far-jump kernel moves
 // everything from the untouched data)
 template <int D, typename T>
 int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* index, int nargs,
@@ -1101,8 +1122,18 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     ms.stg = base + o_stg;
 
     int* flag = dflags(ctx);
-    // the far-jump flag is zero here: a new context starts with it cleared,
-    // and the serial fallback, the last kernel of a move, clears it after use
+    // the flag words are zero here: a new context starts with them cleared,
+    // the far-jump kernel, the last of a move, clears them after use, and a
+    // launch that fails after classify clears them here
+    auto launched = [&]() -> int {
+        const hipError_t e = hipGetLastError();
+        if (e == hipSuccess) return JP_OK;
+        if (!ctx->capturing) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipMemset(flag, 0, 2 * sizeof(int));
+        }
+        return fail(ctx, JP_ERR_HIP, "HIP error %s", hipGetErrorString(e));
+    };
     if ((rc = stats_begin(ctx, stats))) return rc;
     jp_stats* st = stats ? dstats(ctx) : nullptr;
     ArgPtrs ap = argptrs(args, nargs);
@@ -1122,7 +1153,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
         k_mv_classify_split<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
                                                                  flag, span, geo_ptr(ctx));
     }
-    JP_LAUNCHED(ctx);
+    if ((rc = launched())) return rc;
     int64_t total = 1;
     for (int d = 0; d < D; ++d) total *= (g.ni[d] + 2) / 3;
     constexpr int BS = D == 2 ? 128 : 64;  // measured: 256 and 64 (2D), 128 (3D) slower
@@ -1140,7 +1171,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
         else
             k_mv_tiles2<TX, TY, 256, false><<<(unsigned)tiles, 256, 0, ctx->stream>>>(g2, ms,
                                                                                     flag, st);
-        JP_LAUNCHED(ctx);
+        if ((rc = launched())) return rc;
     }
     for (int ph = 0; ph < nphase; ++ph) {
         Off3 o{{0, 0, 0}};
@@ -1151,13 +1182,13 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
         }
         k_mv_phase<D, BS><<<(unsigned)((total + BS - 1) / BS), BS, 0, ctx->stream>>>(g, ms, o, ph,
                                                                                     flag, st);
-        JP_LAUNCHED(ctx);
+        if ((rc = launched())) return rc;
     }
     k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, flag,
                                                          span);
-    JP_LAUNCHED(ctx);
-    k_mv_serial<D, T><<<1, 64, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
-    JP_LAUNCHED(ctx);
+    if ((rc = launched())) return rc;
+    k_mv_far<D, T><<<1, 1024, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
+    if ((rc = launched())) return rc;
     if ((rc = stats_end(ctx, stats))) return rc;
     return std::is_same<T, double>::value ? finish(ctx) : split_finish(ctx);
 }
