@@ -21,13 +21,13 @@
 //      into their final slot, vacated slots get NaN and mask false.
 //
 // A particle that jumps more than one cell breaks the colour independence;
-// classify raises a flag (and records the largest jump), the phases and the
-// scatter return immediately, and a one-workgroup kernel, launched last, runs
-// the reference loop on the particle data for every phase, cells in levels
-// that keep the reference's order wherever two cells interact, and clears
-// the flag.  The split classify raises another bit of the same flag when the
-// grid's geometry was rejected: then nothing moves and the last kernel only
-// clears the flag.
+// classify raises a flag (and records the largest jump), the regular phases
+// return immediately, and a one-workgroup kernel runs the same slot loops on
+// the same words, cell by cell in levels that keep the reference's order
+// wherever two cells interact, finding the far destinations from the staged
+// positions, and clears the flag; the scatter then moves the data as usual.
+// The split classify raises another bit of the same flag when the grid's
+// geometry was rejected: then nothing moves.
 #include "jp_ctx.hpp"
 
 #include <type_traits>
@@ -37,6 +37,10 @@ using namespace jp;
 namespace {
 
 constexpr uint8_t kDel = 0xFE;
+// a mover whose destination is found from its staged position when its cell
+// is processed (far-jump path only: movers that jump more than one cell, and
+// arrivals there that fail the strict test of their new cell)
+constexpr uint8_t kFar = 0xFC;
 constexpr int kFlagFar = 1;       // a mover jumps more than one cell
 constexpr int kFlagRejected = 2;  // split geometry rejected
 constexpr uint8_t kGap = 0x80;  // arrival fails the strict in-cell test of its destination
@@ -245,17 +249,17 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     const double h = r < 0 ? hx[d][0] : (r == 0 ? hx[d][1] : hx[d][2]);
                     inJ = inJ & (lo < p[d] && p[d] < lo + h);
                 }
-                if (far) {
+                if (far) {  // the far-jump path finds the destination again
                     atomicOr(flag, kFlagFar);
                     atomicMax(flag + 1, (int)(jump < (1 << 30) ? jump : (1 << 30)));
-                    cd = kDel;  // unused: the far-jump kernel takes over
+                    cd = kFar;
                 } else {
                     cd = (uint8_t)n;
                     if (n != kCenter<D> && !inJ) cd |= kGap;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, c + C * k, d) = p[d];
-                    stg |= 1u << j;
                 }
+#pragma unroll
+                for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, c + C * k, d) = p[d];
+                stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
         }
@@ -292,31 +296,16 @@ struct alignas(16) NbEntry {
     uint32_t pad;
 };
 
-// Colour cell t of phase `phase` (offsets `off`): the reference slot loop of
-// that cell.  `stop`: the far-jump kernel runs instead (checked after the
-// cell's loads are issued).
-template <int D, int BS>
-__device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const Off3& off,
-                                  int phase, int64_t t, bool stop, NbEntry (&s_nb)[kNB<D>][BS],
+// The reference slot loop of cell I in phase `phase`.  `stop`: the far-jump
+// path runs instead (checked after the cell's loads are issued).  FAR (the
+// far-jump path, cells in an order that keeps apart any two that interact):
+// kFar codes are resolved here from the staged positions, and a destination
+// beyond the 3^D neighbourhood is updated in the global words directly.
+template <int D, int BS, typename T, bool FAR>
+__device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const int64_t* I,
+                                  int phase, bool stop, NbEntry (&s_nb)[kNB<D>][BS],
                                   uint64_t (&s_cw)[8][BS], int& n_moved, int& n_del,
-                                  int& n_drop) {
-    int64_t ncol[D];
-    int64_t total = 1;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        ncol[d] = (g.ni[d] + 2) / 3;
-        total *= ncol[d];
-    }
-    if (t >= total) return;
-    int64_t I[D];
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        I[d] = 3 * (t % ncol[d]) + off.o[d];
-        t /= ncol[d];
-        ok = ok && I[d] < g.ni[d];
-    }
-    if (!ok) return;
+                                  int& n_drop, int& n_far) {
     const int64_t S = lin<D>(g.ni, I);
     const int tid = threadIdx.x;
     const int K = g.K;
@@ -375,13 +364,71 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
             // and stores) for the previous iteration's store
             __builtin_amdgcn_s_waitcnt(kVmcnt0);
         }
-        const uint8_t cd = (selfm & bit) ? (uint8_t)kCenter<D> : code;
+        uint8_t cd = (selfm & bit) ? (uint8_t)kCenter<D> : code;
         occC &= ~bit;  // empty_particle! on the source slot
         arrS &= ~bit;
         selfm &= ~bit;
         if (cd == kDel) {
             ++n_del;
             continue;
+        }
+        if constexpr (FAR) {
+            if (cd == kFar) {
+                // the destination from the staged position: bisection seeded
+                // at this cell (FP64), the cell + floor(u) (split, whose staged
+                // offset is renormalised to it here, for the scatter)
+                int64_t J[D];
+                bool near = true, inJ = true;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const T q = stg_coord<D, T>(ms, origin, d);
+                    if constexpr (std::is_same<T, float>::value) {
+                        const int sd = (int)floorf(q);
+                        J[d] = I[d] + sd;
+                        stg_coord<D, T>(ms, origin, d) = (float)((double)q - (double)sd);
+                    } else {
+                        J[d] = find_cell(q, g.xv[d], I[d]);
+                        const double xl = g.xv[d].x[J[d]];
+                        inJ = inJ && xl < q && q < xl + g.xv[d].dx[J[d]];
+                    }
+                    near = near && J[d] - I[d] <= 1 && I[d] - J[d] <= 1;
+                }
+                if (near) {  // inside the neighbourhood: the usual path
+                    int nn = 0, mult = 1;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        nn += (int)(J[d] - I[d] + 1) * mult;
+                        mult *= 3;
+                    }
+                    cd = (uint8_t)nn;
+                    if (nn != kCenter<D> && !inJ) cd |= kGap;
+                } else {
+                    ++n_far;
+                    const int64_t dcn = lin<D>(g.ni, J);
+                    const uint64_t od = ms.occ[dcn];
+                    const uint64_t fr = ~od & full & (~0ull << sp);
+                    if (!fr) {
+                        ++n_drop;
+                        continue;
+                    }
+                    const int f = __builtin_ctzll(fr);
+                    sp = f;
+                    const uint64_t fb = 1ull << f;
+                    ++n_moved;
+                    ms.org[org_at<D>((uint32_t)dcn, (uint32_t)f, C32, (uint32_t)K)] = origin;
+                    ms.occ[dcn] = od | fb;
+                    atomicOr(reinterpret_cast<unsigned long long*>(ms.arr + dcn),
+                             (unsigned long long)fb);
+                    atomicOr(reinterpret_cast<unsigned long long*>(ms.tch + dcn),
+                             (unsigned long long)fb);
+                    if (!inJ && phase_of<D>(J) > phase) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(ms.mov + dcn),
+                                 (unsigned long long)fb);
+                        ms.code[((f >> 3) * C + dcn) * 8 + (f & 7)] = kFar;
+                    }
+                    continue;
+                }
+            }
         }
         const int n = cd & 0x7F;
         const NbEntry nb = s_nb[n][tid];
@@ -416,7 +463,9 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
             unlin<D>(g.ni, Dc, J);
             if (phase_of<D>(J) > phase) {
                 ms.mov[Dc] |= fb;
-                ms.code[((f >> 3) * C + Dc) * 8 + (f & 7)] = (uint8_t)kCenter<D>;
+                // (the far-jump path looks the destination up again, as the
+                // reference does)
+                ms.code[((f >> 3) * C + Dc) * 8 + (f & 7)] = FAR ? kFar : (uint8_t)kCenter<D>;
             }
         }
     }
@@ -457,9 +506,23 @@ __global__ void __launch_bounds__(BS) k_mv_phase(Grid<D> g, MoveScratch ms, Off3
     __shared__ NbEntry s_nb[kNB<D>][BS];
     __shared__ uint64_t s_cw[8][BS];
     const int stop = *flag;
-    int n_moved = 0, n_del = 0, n_drop = 0;
-    phase_cell<D, BS>(g, ms, off, phase, blockIdx.x * (int64_t)BS + threadIdx.x, stop != 0, s_nb,
-                      s_cw, n_moved, n_del, n_drop);
+    int n_moved = 0, n_del = 0, n_drop = 0, n_far = 0;
+    int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x, total = 1, ncol[D], I[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ncol[d] = (g.ni[d] + 2) / 3;
+        total *= ncol[d];
+    }
+    bool ok = t < total;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I[d] = 3 * (t % ncol[d]) + off.o[d];
+        t /= ncol[d];
+        ok = ok && I[d] < g.ni[d];
+    }
+    if (ok)
+        phase_cell<D, BS, double, false>(g, ms, I, phase, stop != 0, s_nb, s_cw, n_moved, n_del,
+                                         n_drop, n_far);
     stat_add(st, ST_MOVED, n_moved);
     stat_add(st, ST_DELETED, n_del);
     stat_add(st, ST_DROPPED, n_drop);
@@ -698,13 +761,15 @@ template <int D, typename T>
 __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> coords,
                                                        uint8_t* __restrict__ index, int nargs,
                                                        ArgPtrs args, MoveScratch ms,
-                                                       const int* __restrict__ flag, int span) {
+                                                       const SplitGeom* __restrict__ geo,
+                                                       int span) {
     // one thread per cell and slot group (blockIdx.y, `span` slots), chunks of
     // 8 slot planes in lockstep across the wavefront (coalesced origin loads
     // and final stores); only the reads of the staged data are scattered.
     // Arrivals copy their staged data, vacated slots are emptied
-    // (empty_particle!, src/Particles/move_safe.jl:243-251).
-    if (*flag) return;
+    // (empty_particle!, src/Particles/move_safe.jl:243-251).  Nothing moves
+    // when the split grid was rejected (geo: null in the FP64 form).
+    if (geo && geo->bad) return;
     // 2D: XCD-contiguous cell ranges (a destination's sources are its
     // neighbour cells, whose staged lines other threads of the same L2 also
     // read; a row of cells above or below is only a few workgroups away, which
@@ -773,126 +838,10 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
 }
 
 // ------------------------------------------------------------ far jumps
-// move_kernel! (src/Particles/move_safe.jl:72-125) for one cell, directly on
-// the data (the far-jump path).  The cell's occupancy and, per mover, the
-// destination's are read as one round of independent byte loads, and the
-// slot loop runs on those words: the slots are visited in order (a particle
-// re-slotted higher in its own cell is visited again, as in the reference),
-// the destination slot is the first free one at or after starting_point.
-// FP64 positions: isincell / indomain, then the bisection seeded at the
-// cell; split offsets: the destination is the cell + floor(u) and the offset
-// is renormalised to it.
-template <int D, typename T>
-__device__ void move_cell_far(const Grid<D>& g, const TPtr3<D, T>& coords, uint8_t* index,
-                              int nargs, const ArgPtrs& args, const int64_t* I,
-                              int64_t (&cnt)[4]) {
-    constexpr bool SPLIT = std::is_same<T, float>::value;
-    const int64_t C = g.C;
-    const int K = g.K;
-    const int64_t c = lin<D>(g.ni, I);
-    const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
-    double xlo[D], hx[D], dlo[D], dhi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        if constexpr (!SPLIT) {
-            const Axis& A = g.xv[d];
-            xlo[d] = A.x[I[d]];
-            hx[d] = A.dx[I[d]];
-            dlo[d] = A.x[0];
-            dhi[d] = A.x[A.n - 1];
-        } else {
-            xlo[d] = hx[d] = dlo[d] = dhi[d] = 0.0;
-        }
-    }
-    uint64_t own = active_slots(index, c, C, K);
-    uint64_t todo = own;
-    int sp = 0;
-    while (todo) {
-        const int ip = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const int64_t e = c + C * ip;
-        T p[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-        int64_t J[D];
-        bool out = false, far = false;
-        if constexpr (SPLIT) {
-            bool stay = true;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const bool fin = fabsf(p[d]) < 1.0e6f;  // false for NaN/Inf
-                const int sd = fin ? (int)floorf(p[d]) : 0;
-                out = out || !fin;
-                J[d] = I[d] + sd;
-                stay = stay && sd == 0;
-                far = far || sd > 1 || sd < -1;
-                out = out || J[d] < 0 || J[d] >= g.ni[d];
-            }
-            if (stay && !out) continue;
-        } else {
-            bool inc = true, indom = true;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                inc = inc & (xlo[d] < p[d] && p[d] < xlo[d] + hx[d]);  // isincell
-                indom = indom & (dlo[d] < p[d] && p[d] < dhi[d]);      // indomain
-            }
-            if (inc) continue;
-            out = !indom;
-            if (!out) {
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    J[d] = find_cell(p[d], g.xv[d], I[d]);
-                    far = far || J[d] - I[d] > 1 || I[d] - J[d] > 1;
-                }
-            }
-        }
-        const uint64_t bit = 1ull << ip;
-        own &= ~bit;
-        if (out) {  // leaves the domain: emptied
-            index[e] = 0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) coords.p[d][e] = (T)dnan();
-            for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-            cnt[1]++;
-            continue;
-        }
-        if (far) cnt[3]++;
-        double cached[JP_MAX_ARGS];
-        for (int a = 0; a < nargs; ++a) cached[a] = args.p[a][e];
-        index[e] = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][e] = (T)dnan();
-        for (int a = 0; a < nargs; ++a) args.p[a][e] = dnan();
-        const int64_t cn = lin<D>(g.ni, J);
-        const uint64_t dm = cn == c ? own : active_slots(index, cn, C, K);
-        const uint64_t fr = ~dm & full & (~0ull << sp);
-        if (!fr) {  // find_free_memory found nothing: dropped
-            cnt[2]++;
-            continue;
-        }
-        const int fi = __builtin_ctzll(fr);
-        sp = fi;  // starting_point = free_idx
-        const int64_t f = cn + C * fi;
-        index[f] = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            if constexpr (SPLIT)
-                coords.p[d][f] = (float)((double)p[d] - (double)(J[d] - I[d]));
-            else
-                coords.p[d][f] = p[d];
-        }
-        for (int a = 0; a < nargs; ++a) args.p[a][f] = cached[a];
-        cnt[0]++;
-        if (cn == c) {  // re-slotted in its own cell
-            own |= 1ull << fi;
-            if (fi > ip) todo |= 1ull << fi;
-        }
-    }
-}
-
-// The reference loop of every cell, phase after phase, when some particle
-// jumps more than one cell (flag[0] has kFlagFar, flag[1] holds the largest
-// jump R in cells).  One workgroup: a cell's processing touches only cells
+// The colour phases when some particle jumps more than one cell (flag[0] has
+// kFlagFar, flag[1] holds the largest jump R in cells), in one workgroup on
+// the same words, codes and staged data as the regular phases; the scatter
+// then moves the data as usual.  A cell's processing touches only cells
 // within R of it, so two cells of one colour phase (3t + o) interact only
 // when their colour coordinates t differ by at most q = floor(2R/3) in every
 // dimension.  Within a phase the reference visits the cells in linear order
@@ -903,72 +852,75 @@ __device__ void move_cell_far(const Grid<D>& g, const TPtr3<D, T>& coords, uint8
 // barriers and the result is the reference's.  With a small R that is
 // 3^D * (about (1 + m_1 + m_2) * the colour count per dimension) levels; a
 // jump of the grid's size makes every level one cell (the serial order).
-// Also clears the flag words after a rejected split grid.
+constexpr int kFarBlock = 256;
+
 template <int D, typename T>
-__global__ void __launch_bounds__(1024) k_mv_far(Grid<D> g, TPtr3<D, T> coords, uint8_t* index,
-                                                 int nargs, ArgPtrs args, int* __restrict__ flag,
-                                                 jp_stats* st) {
-    const int fl = *flag;
+__global__ void __launch_bounds__(kFarBlock) k_mv_far(Grid<D> g, MoveScratch ms,
+                                                      int* __restrict__ flag, jp_stats* st) {
+    __shared__ NbEntry s_nb[kNB<D>][kFarBlock];
+    __shared__ uint64_t s_cw[8][kFarBlock];
+    const int fl = flag[0];
     if (!fl) return;
     const int R = flag[1];
     __syncthreads();  // every thread has read the words before they are cleared
-    if (!(fl & kFlagRejected)) {
-        int64_t ncol[D], m[D];
+    if (!(fl & kFlagFar) || (fl & kFlagRejected)) {
+        if (threadIdx.x == 0) flag[0] = flag[1] = 0;
+        return;
+    }
+    int64_t ncol[D], m[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) ncol[d] = (g.ni[d] + 2) / 3;
-        const int64_t q = (2 * (int64_t)R) / 3;
-        int64_t lmax = 0;
+    for (int d = 0; d < D; ++d) ncol[d] = (g.ni[d] + 2) / 3;
+    const int64_t q = (2 * (int64_t)R) / 3;
+    int64_t lmax = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            m[d] = 1;
-            for (int e = 0; e < d; ++e) m[d] += m[e] * (q < ncol[e] - 1 ? q : ncol[e] - 1);
-            lmax += m[d] * (ncol[d] - 1);
+    for (int d = 0; d < D; ++d) {
+        m[d] = 1;
+        for (int e = 0; e < d; ++e) m[d] += m[e] * (q < ncol[e] - 1 ? q : ncol[e] - 1);
+        lmax += m[d] * (ncol[d] - 1);
+    }
+    // the (t_1, ..., t_{D-1}) combinations, one per thread and round
+    int64_t outer = 1;
+#pragma unroll
+    for (int d = 1; d < D; ++d) outer *= ncol[d];
+    int n_moved = 0, n_del = 0, n_drop = 0, n_far = 0;
+    for (int ph = 0; ph < kNB<D>; ++ph) {
+        int o[D];
+        int rr = ph;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            o[d] = rr % 3;
+            rr /= 3;
         }
-        // the (t_1, ..., t_{D-1}) combinations, one per thread and round
-        int64_t outer = 1;
+        for (int64_t lv = 0; lv <= lmax; ++lv) {
+            for (int64_t i = threadIdx.x; i < outer; i += kFarBlock) {
+                int64_t t[D], rest = i, l0 = lv;
 #pragma unroll
-        for (int d = 1; d < D; ++d) outer *= ncol[d];
-        int64_t cnt[4] = {0, 0, 0, 0};
-        for (int ph = 0; ph < kNB<D>; ++ph) {
-            int o[D];
-            int rr = ph;
-#pragma unroll
-            for (int d = D - 1; d >= 0; --d) {
-                o[d] = rr % 3;
-                rr /= 3;
-            }
-            for (int64_t lv = 0; lv <= lmax; ++lv) {
-                for (int64_t i = threadIdx.x; i < outer; i += blockDim.x) {
-                    int64_t t[D], rest = i, l0 = lv;
-#pragma unroll
-                    for (int d = 1; d < D; ++d) {
-                        t[d] = rest % ncol[d];
-                        rest /= ncol[d];
-                        l0 -= m[d] * t[d];
-                    }
-                    t[0] = l0;
-                    if (t[0] < 0 || t[0] >= ncol[0]) continue;
-                    int64_t I[D];
-                    bool ok = true;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        I[d] = 3 * t[d] + o[d];
-                        ok = ok && I[d] < g.ni[d];
-                    }
-                    if (ok) move_cell_far<D, T>(g, coords, index, nargs, args, I, cnt);
+                for (int d = 1; d < D; ++d) {
+                    t[d] = rest % ncol[d];
+                    rest /= ncol[d];
+                    l0 -= m[d] * t[d];
                 }
-                __syncthreads();
+                t[0] = l0;
+                if (t[0] < 0 || t[0] >= ncol[0]) continue;
+                int64_t I[D];
+                bool ok = true;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    I[d] = 3 * t[d] + o[d];
+                    ok = ok && I[d] < g.ni[d];
+                }
+                if (ok)
+                    phase_cell<D, kFarBlock, T, true>(g, ms, I, ph, false, s_nb, s_cw, n_moved,
+                                                      n_del, n_drop, n_far);
             }
+            __syncthreads();
         }
-        stat_add(st, ST_MOVED, cnt[0]);
-        stat_add(st, ST_DELETED, cnt[1]);
-        stat_add(st, ST_DROPPED, cnt[2]);
-        stat_add(st, ST_FAR, cnt[3]);
     }
-    if (threadIdx.x == 0) {  // ready for the next move on this context
-        flag[0] = 0;
-        flag[1] = 0;
-    }
+    stat_add(st, ST_MOVED, n_moved);
+    stat_add(st, ST_DELETED, n_del);
+    stat_add(st, ST_DROPPED, n_drop);
+    stat_add(st, ST_FAR, n_far);
+    if (threadIdx.x == 0) flag[0] = flag[1] = 0;  // ready for the next move
 }
 
 // ------------------------------------------------------------ split classify
@@ -1039,10 +991,13 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
             uint8_t cd;
             if (out) {
                 cd = kDel;
-            } else if (far) {
+            } else if (far) {  // staged as it is: the far-jump path renormalises it
                 atomicOr(flag, kFlagFar);
                 atomicMax(flag + 1, jump);
-                cd = kDel;  // unused: the far-jump kernel takes over
+                cd = kFar;
+#pragma unroll
+                for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, c + C * k, d) = ub[j][d];
+                stg |= 1u << j;
             } else {
                 cd = (uint8_t)n;
 #pragma unroll
@@ -1079,9 +1034,10 @@ TPtr3<D, T> tptr3(T* const* c) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// classify -> 3^D colour phases -> scatter -> far-jump kernel (after a far
-// jump the phases and the scatter return at once, and the far-jump kernel moves
-// everything from the untouched data)
+// classify -> colour phases (2D: one tiled launch; 3D: 3^D launches) -> far-
+// jump phases (one workgroup; returns at once unless a particle jumped more
+// than one cell, in which case the regular phases returned at once) ->
+// scatter
 template <int D, typename T>
 int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* index, int nargs,
              double* const* args, jp_stats* stats) {
@@ -1123,8 +1079,9 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
 
     int* flag = dflags(ctx);
     // the flag words are zero here: a new context starts with them cleared,
-    // the far-jump kernel, the last of a move, clears them after use, and a
-    // launch that fails after classify clears them here
+    // the far-jump kernel, which runs on every move after the regular phases,
+    // clears them after use, and a launch that fails after classify clears
+    // them here
     auto launched = [&]() -> int {
         const hipError_t e = hipGetLastError();
         if (e == hipSuccess) return JP_OK;
@@ -1184,11 +1141,13 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
                                                                                     flag, st);
         if ((rc = launched())) return rc;
     }
-    k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, flag,
+    k_mv_far<D, T><<<1, kFarBlock, 0, ctx->stream>>>(g, ms, flag, st);
+    if ((rc = launched())) return rc;
+    const SplitGeom* geo = std::is_same<T, double>::value ? nullptr : geo_ptr(ctx);
+    k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, geo,
                                                          span);
     if ((rc = launched())) return rc;
-    k_mv_far<D, T><<<1, 1024, 0, ctx->stream>>>(g, cp, index, nargs, ap, flag, st);
-    if ((rc = launched())) return rc;
+
     if ((rc = stats_end(ctx, stats))) return rc;
     return std::is_same<T, double>::value ? finish(ctx) : split_finish(ctx);
 }

@@ -1,8 +1,10 @@
 """Moves in which particles jump more than one cell (runs on an MI355X).
 
-Such a move runs the reference loop cell by cell in one workgroup, in levels
-that keep the reference's order between any two cells that can interact (the
-level spacing follows the largest jump).  Compared with the oracle bit for
+Such a move runs the colour phases' slot loops in one workgroup, cell by cell
+in levels that keep the reference's order between any two cells that can
+interact (the level spacing follows the largest jump), finding the far
+destinations from the staged positions; the scatter then moves the data as
+in a regular move.  Compared with the oracle bit for
 bit: FP64 and FP32-split, 2D and 3D, jumps of 2 to 20 cells, crowded cells
 (drops), particles leaving the domain (deletions), and a regular move on the
 same context afterwards (the far-jump state is cleared after use).

@@ -1,7 +1,7 @@
 """The level order of the far-jump move (CPU).
 
-When some particle jumps R > 1 cells, the move runs the reference loop cell
-by cell in one workgroup.  In each colour phase the cells (colour coordinates
+When some particle jumps R > 1 cells, the move runs the colour phases' slot
+loops cell by cell in one workgroup.  In each colour phase the cells (colour coordinates
 t) go in levels L(t) = sum_d m_d t_d, m_0 = 1, m_d = 1 + sum_{e<d} m_e q_e,
 q_e = min(floor(2R/3), ncol_e - 1), levels 0 .. sum_d m_d (ncol_d - 1), the
 t_0 of a level found from the other coordinates.  Two cells interact when
