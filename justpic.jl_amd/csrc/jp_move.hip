@@ -421,7 +421,13 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
                              (unsigned long long)fb);
                     atomicOr(reinterpret_cast<unsigned long long*>(ms.tch + dcn),
                              (unsigned long long)fb);
-                    if (!inJ && phase_of<D>(J) > phase) {
+                    // an arrival that fails the strict test of its new cell is
+                    // looked up again when that cell comes later: in a later
+                    // phase, or later in this one (a jump by a multiple of 3
+                    // cells keeps the colour; cells of a phase go in linear
+                    // order)
+                    const int pj = phase_of<D>(J);
+                    if (!inJ && (pj > phase || (pj == phase && dcn > S))) {
                         atomicOr(reinterpret_cast<unsigned long long*>(ms.mov + dcn),
                                  (unsigned long long)fb);
                         ms.code[((f >> 3) * C + dcn) * 8 + (f & 7)] = kFar;

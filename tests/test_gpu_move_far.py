@@ -26,11 +26,12 @@ def say(*a):
     print("[gpu-move-far]", *a, flush=True)
 
 
-def jumped_state(ni, K, fill, far_frac, R, seed, dyadic=True):
+def jumped_state(ni, K, fill, far_frac, R, seed, on_vertex=0.0):
     """Particles in a fraction `fill` of the slots, displaced by -1..1 cells,
     and a fraction far_frac of them by up to R cells per dimension (some
     beyond the domain).  Offsets are odd multiples of 1/64 of a cell (exact
-    in float32 on power-of-two grids)."""
+    in float32 on power-of-two grids), or with probability on_vertex exactly
+    0 (the particle sits on a vertex line)."""
     D = len(ni)
     xv, xi = grid2d(*ni) if D == 2 else grid3d(ni[0])
     hp = O.HostParticles(xv, K, xi)
@@ -42,6 +43,8 @@ def jumped_state(ni, K, fill, far_frac, R, seed, dyadic=True):
     far = rng.random(hp.S) < far_frac
     for d in range(D):
         u = (2 * rng.integers(0, 32, hp.S) + 1) / 64.0
+        if on_vertex:
+            u = np.where(rng.random(hp.S) < on_vertex, 0.0, u)
         shift = np.where(far, rng.integers(-R, R + 1, hp.S), rng.choice([-1, 0, 0, 1], hp.S))
         x = (I[d] + u + shift) / ni[d]
         hp.coords[d][:] = np.where(occ, x, np.nan)
@@ -96,6 +99,27 @@ def test_far_move_split(ctx, case):
     for d in range(len(ni)):
         assert np.array_equal(coords[d], hp.coords[d], equal_nan=True)
     assert np.array_equal(Td.data.flat(), ids, equal_nan=True)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_far_move_vertex_lines(ctx, D):
+    """FP64 far jumps onto vertex lines: a particle that lands on the edge of
+    its destination fails the strict test there and is looked up again when
+    that cell's phase comes later (bit for bit against the oracle)."""
+    say("far vertex", D)
+    ni = (32, 32) if D == 2 else (8, 8, 8)
+    hp = jumped_state(ni, 12, 0.6, 0.1, 3, seed=11 + D, on_vertex=0.25)
+    ids = np.where(hp.index == 1, np.arange(hp.S, dtype=np.float64), np.nan)
+    p = to_dev(ctx, hp)
+    Td = jp.CellArray(ctx, hp.ni, hp.K)
+    Td.copy_from(ids)
+    st_d = jp.move_particles_(p, (Td,), stats=True)
+    st_o = O.move_particles(hp, [ids])
+    say("stats", st_d, st_o)
+    assert st_o["far"] > 0
+    assert st_d == st_o
+    assert_state_equal(p, hp, exact=True)
+    assert_field(Td.data.flat(), ids, exact=True)
 
 
 def test_regular_move_after_far_move(ctx):
