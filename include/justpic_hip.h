@@ -94,7 +94,8 @@ void* jp_ctx_stream(jp_ctx* ctx);
  * problems can run the shapes large ones use (the tests compare every
  * setting with the oracle).
  *   JP_OPT_SLOT_SPAN    slots per slot group of the per-cell kernels (move
- *                       classify and scatter, FP32-split advection): 0
+ *                       classify and scatter, FP32-split advection, FP64
+ *                       advection with the linear interpolation): 0
  *                       (default) chooses from the grid size; otherwise a
  *                       multiple of 8 up to 64 (>= max_xcell: whole cells).
  *   JP_OPT_INJECT_WAVES persistent wavefronts per injection colour launch: 0
@@ -106,12 +107,17 @@ void* jp_ctx_stream(jp_ctx* ctx);
  *   JP_OPT_MOVE_PHASES  the 2D move's colour phases: 0 (default) all nine in
  *                       one launch over tiles of cells, 1 one launch per
  *                       phase (the form the 3D move always takes).
+ *   JP_OPT_ADVECT_SLOTS the FP64 advection with the linear interpolation: 0
+ *                       (default) tiles of cells with the velocity staged
+ *                       in LDS, 1 one thread per slot gathering from global
+ *                       memory (the form the MQS and LinP schemes take).
  * No reference counterpart (the KernelAbstractions launch sizes of the
  * reference are chosen by its backend). */
 #define JP_OPT_SLOT_SPAN 1
 #define JP_OPT_INJECT_WAVES 2
 #define JP_OPT_GEOMETRY_RECHECK 3
 #define JP_OPT_MOVE_PHASES 4
+#define JP_OPT_ADVECT_SLOTS 5
 int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value);
 int jp_sync(jp_ctx* ctx);
 int jp_malloc(jp_ctx* ctx, void** dptr, size_t bytes);

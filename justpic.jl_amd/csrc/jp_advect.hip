@@ -4,13 +4,14 @@
 // (backtracking.jl, backtracking_MQS.jl, backtracking_LinP.jl) on gfx950, FP64
 // drop-in, 2D and 3D, Euler / RK2 / RK4.
 //
-// One thread per slot; the velocity components are gathered from global
-// memory, where the few MB of a velocity grid stay L2/MALL-resident.  A bracket
-// lookup costs one guess (uniform spacing estimate) and one strict test against
-// the stored axis, with the reference search for everything else (ties,
-// refined axes).  An LDS-tiled variant (velocity and axis windows per tile of
-// cells, a thread per cell walking its slots) was as fast in 2D and half as
-// fast in 3D, where its register footprint left one wave per SIMD.
+// The linear scheme (advection!): one workgroup per tile of cells with each
+// velocity component's window of nodes, and its axis windows, staged in LDS;
+// one thread per cell walks the cell's slots (k_advect_tile).  A stage point
+// outside a window sends the particle's whole step to the slot path.  The
+// slot path (k_advect_slot, and the MQS / LinP schemes): one thread per
+// slot, the velocity gathered from global memory.  A bracket lookup costs one
+// guess (uniform spacing estimate) and one strict test against the stored
+// axis, with the reference search for everything else (ties, refined axes).
 #include "jp_ctx.hpp"
 
 using namespace jp;
@@ -277,15 +278,15 @@ __device__ inline void interp_v_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, c
     for (int c = 0; c < D; ++c) v[c] = interp_slot<D, S, LIM>(Gs, vg, V, c, p, I);
 }
 
-// advect_particle (Euler.jl, RK2.jl, RK4.jl) with slot-kernel interpolation;
-// the semi-Lagrangian backtracking (advect_particle_SML, RK2.jl:53-107,
-// RK4.jl:44-87) is the same with P.dt = -dt and P.adt = (-alpha) dt
-template <int D, int M, int S = kLinear, bool LIM = true>
-__device__ inline void advect_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, const VPtrs<D>& V,
-                                   const RKParams& P, const double (&p0)[D],
-                                   const int64_t (&I)[D], double (&out)[D]) {
+// advect_particle (Euler.jl, RK2.jl, RK4.jl) for any velocity interpolation
+// interp(p, v); the semi-Lagrangian backtracking (advect_particle_SML,
+// RK2.jl:53-107, RK4.jl:44-87) is the same with P.dt = -dt and
+// P.adt = (-alpha) dt
+template <int D, int M, class Interp>
+__device__ inline void advect_rk(const RKParams& P, const double (&p0)[D], double (&out)[D],
+                                 Interp&& interp) {
     double v0[D];
-    interp_v_slot<D, S, LIM>(Gs, vg, V, p0, I, v0);
+    interp(p0, v0);
     if constexpr (M == JP_EULER) {
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v0[d], p0[d]);
@@ -293,7 +294,7 @@ __device__ inline void advect_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, con
         double p1[D], v1[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) p1[d] = fma(P.adt, v0[d], p0[d]);
-        interp_v_slot<D, S, LIM>(Gs, vg, V, p1, I, v1);  // same seed I (RK2.jl:14,20)
+        interp(p1, v1);  // same seed I (RK2.jl:14,20)
         if (P.half) {
 #pragma unroll
             for (int d = 0; d < D; ++d) out[d] = fma(P.dt, v1[d], p0[d]);
@@ -305,22 +306,32 @@ __device__ inline void advect_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, con
         double q[D], k[D], sum[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = p0[d] + (P.dt * v0[d]) / 2.0;
-        interp_v_slot<D, S, LIM>(Gs, vg, V, q, I, k);
+        interp(q, k);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             sum[d] = v0[d] + 2.0 * k[d];
             q[d] = p0[d] + (P.dt * k[d]) / 2.0;
         }
-        interp_v_slot<D, S, LIM>(Gs, vg, V, q, I, k);
+        interp(q, k);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             sum[d] = sum[d] + 2.0 * k[d];
             q[d] = p0[d] + P.dt * k[d];
         }
-        interp_v_slot<D, S, LIM>(Gs, vg, V, q, I, k);
+        interp(q, k);
 #pragma unroll
         for (int d = 0; d < D; ++d) out[d] = p0[d] + (P.dt * (sum[d] + k[d])) / 6.0;
     }
+}
+
+// with the slot-kernel interpolation (velocity gathered from global memory)
+template <int D, int M, int S = kLinear, bool LIM = true>
+__device__ inline void advect_slot(const AxisGuess<D>& Gs, const Vel<D>& vg, const VPtrs<D>& V,
+                                   const RKParams& P, const double (&p0)[D],
+                                   const int64_t (&I)[D], double (&out)[D]) {
+    advect_rk<D, M>(P, p0, out, [&](const double (&p)[D], double (&v)[D]) {
+        interp_v_slot<D, S, LIM>(Gs, vg, V, p, I, v);
+    });
 }
 
 template <int D>
@@ -353,6 +364,193 @@ __global__ void __launch_bounds__(kBlock) k_advect_slot(Grid<D> g, Vel<D> vg, VP
     advect_slot<D, M, S>(Gs, vg, V, P, p0, I, out);
 #pragma unroll
     for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
+}
+
+// The linear scheme with the velocity staged in LDS: one workgroup per tile
+// of cells (2D 64x4, 3D 16x4x4), one thread per cell walking the cell's slot
+// planes (a wavefront covers whole 128-byte runs of a plane).  Each
+// component's window holds the nodes from two below the tile's first bracket
+// to kWinPad - 2 beyond its end, which covers the RK stage points of a step of
+// up to one cell; its axis coordinates and inverse spacings are staged too.
+// A bracket, and the corner values, come from the window when they lie in it
+// and from global memory otherwise, with the same tests and the same values
+// as the slot kernel, so the results are identical.
+template <int D>
+struct F64Tile;
+template <>
+struct F64Tile<2> {
+    static constexpr int n[3] = {64, 4, 1};
+};
+template <>
+struct F64Tile<3> {
+    static constexpr int n[3] = {16, 4, 4};
+};
+constexpr int kWinPad = 6;
+
+template <int D>
+struct F64Win {
+    static constexpr int w(int d) { return d < D ? F64Tile<D>::n[d] + kWinPad : 1; }
+    static constexpr int size() { return w(0) * w(1) * w(2); }
+    static constexpr int axis() { return w(0) > w(1) ? (w(0) > w(2) ? w(0) : w(2)) : (w(1) > w(2) ? w(1) : w(2)); }
+};
+
+// window entries past the end of an axis hold NaN (no bracket test passes)
+template <int D>
+struct TileLds {
+    double v[D][F64Win<D>::size()];
+    double x[D][D][F64Win<D>::axis()];
+    double idx[D][D][F64Win<D>::axis()];
+};
+
+// Component c at p from the window, if every axis brackets p strictly inside
+// it (that bracket is the axis's only one, the one the slot kernel finds, and
+// it lies inside the component's grid); false otherwise.
+template <int D>
+__device__ inline bool interp_tile_f64(const TileLds<D>& L, int c, const double (&p)[D],
+                                       double& v) {
+    int w[D];
+    double t[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double g = (p[d] - L.x[c][d][0]) * L.idx[c][d][0];
+        int k = g >= 0.0 ? (g < (double)(F64Win<D>::w(d) - 2) ? (int)g : F64Win<D>::w(d) - 2) : 0;
+        const double xl = L.x[c][d][k], xr = L.x[c][d][k + 1];
+        if (!(xl < p[d] && p[d] < xr)) return false;
+        w[d] = k;
+        t[d] = (p[d] - xl) * L.idx[c][d][k];
+    }
+    const int b = w[0] + F64Win<D>::w(0) * (w[1] + (D == 3 ? F64Win<D>::w(1) * w[D - 1] : 0));
+    double q[1 << D];
+#pragma unroll
+    for (int r = 0; r < (1 << D); ++r) {
+        const int o = (r & 1) + ((r >> 1) & 1) * F64Win<D>::w(0) +
+                      (D == 3 ? ((r >> 2) & 1) * F64Win<D>::w(0) * F64Win<D>::w(1) : 0);
+        q[r] = L.v[c][b + o];
+    }
+    double vl = lerp1(t[1], lerp1(t[0], q[0], q[1]), lerp1(t[0], q[2], q[3]));
+    if constexpr (D == 3) {
+        const double hi = lerp1(t[1], lerp1(t[0], q[4], q[5]), lerp1(t[0], q[6], q[7]));
+        vl = lerp1(t[2], vl, hi);
+    }
+    v = vl;
+    return true;
+}
+
+template <int D, int M>
+__global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VPtrs<D> V,
+                                                        Ptr3 coords,
+                                                        const uint8_t* __restrict__ index,
+                                                        RKParams P, int span) {
+    using TL = F64Tile<D>;
+    using WN = F64Win<D>;
+    __shared__ TileLds<D> L;
+    int64_t t0[D];
+    {
+        int64_t b = blockIdx.x;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t nt = (g.ni[d] + TL::n[d] - 1) / TL::n[d];
+            t0[d] = (b % nt) * TL::n[d];
+            b /= nt;
+        }
+    }
+    const int kb = blockIdx.y * span;
+    const int ke = kb + span < g.K ? kb + span : g.K;
+    const int tid = threadIdx.x;
+    // window origins: two nodes below the guessed bracket of the tile's first
+    // vertex (only where the window sits depends on the guess, not a result)
+    int64_t wlo[D][D];
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const Axis& A = vg.ax[c][d];
+            const double x = g.xv[d].x[t0[d]];
+            const int64_t last = A.n - 2;
+            int64_t gk = (int64_t)((x - A.x[0]) * A.idx[0]);
+            gk = gk < 0 ? 0 : (gk > last ? last : gk);
+            wlo[c][d] = gk - 2 < 0 ? 0 : gk - 2;
+        }
+    const double qnan = __builtin_nan("");
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const double* __restrict__ F = V.v[c];
+        for (int i = tid; i < WN::size(); i += kBlock) {
+            int r = i;
+            int64_t node = 0, stride = 1;
+            bool ok = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int64_t kk = wlo[c][d] + r % WN::w(d);
+                r /= WN::w(d);
+                ok = ok && kk < vg.ax[c][d].n;
+                node += kk * stride;
+                stride *= vg.ax[c][d].n;
+            }
+            L.v[c][i] = ok ? F[node] : qnan;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            for (int w = tid; w < WN::w(d); w += kBlock) {
+                const Axis& A = vg.ax[c][d];
+                const int64_t kk = wlo[c][d] + w;
+                L.x[c][d][w] = kk < A.n ? A.x[kk] : qnan;
+                L.idx[c][d][w] = kk < A.n - 1 ? A.idx[kk] : qnan;
+            }
+    }
+    __syncthreads();
+    int64_t I[D];
+    bool inside = true;
+    {
+        int r = tid;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            I[d] = t0[d] + r % TL::n[d];
+            r /= TL::n[d];
+            inside = inside && I[d] < g.ni[d];
+        }
+    }
+    if (!inside) return;
+    const int64_t c = lin<D>(g.ni, I);
+    const int64_t C = g.C;
+    // the step from the windows; if any stage point leaves a window (or sits
+    // on a node), the whole step again on the slot kernel's path
+    bool win = true;
+    auto interp = [&](const double (&p)[D], double (&v)[D]) {
+#pragma unroll
+        for (int cc = 0; cc < D; ++cc) win = interp_tile_f64<D>(L, cc, p, v[cc]) && win;
+    };
+    // the next active slot's position is loaded while this one is advected
+    const uint64_t act = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                : active_slots_in(index, c, C, kb, ke);
+    uint64_t todo = act;
+    double pn[D];
+    int kn = todo ? __builtin_ctzll(todo) : 0;
+    if (todo) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) pn[d] = coords.p[d][c + C * kn];
+    }
+    while (todo) {
+        const int k = kn;
+        todo &= todo - 1;
+        double p0[D], out[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p0[d] = pn[d];
+        if (todo) {
+            kn = __builtin_ctzll(todo);
+#pragma unroll
+            for (int d = 0; d < D; ++d) pn[d] = coords.p[d][c + C * kn];
+        }
+        win = true;
+        advect_rk<D, M>(P, p0, out, interp);
+        if (!win) {
+            AxisGuess<D> Gs;
+            axis_guess<D>(vg, Gs);
+            advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) coords.p[d][c + C * k] = out[d];
+    }
 }
 
 // semilagrangian_advection! (backtracking.jl:15-101 and the MQS / LinP forms):
@@ -447,6 +645,19 @@ Vel<D> to_vel(const jp_velgrid* vg) {
 template <int D, int M, int S>
 int launch(jp_ctx* ctx, const Grid<D>& G, const Vel<D>& vg, const VPtrs<D>& V, Ptr3 cp,
            const uint8_t* index, const RKParams& P) {
+    if constexpr (S == kLinear) {
+        if (!ctx->opt_advect_slots) {
+            int64_t tiles = 1;
+            for (int d = 0; d < D; ++d)
+                tiles *= (G.ni[d] + F64Tile<D>::n[d] - 1) / F64Tile<D>::n[d];
+            // slot groups when the tiles alone leave the chip under-filled
+            const int span = slot_span(ctx, tiles, G.K, 2048);
+            const dim3 grid((unsigned)tiles, (unsigned)((G.K + span - 1) / span));
+            k_advect_tile<D, M><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P, span);
+            JP_LAUNCHED(ctx);
+            return JP_OK;
+        }
+    }
     dim3 grid((unsigned)row_blocks<D>(G), (unsigned)G.K);
     k_advect_slot<D, M, S><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P);
     JP_LAUNCHED(ctx);

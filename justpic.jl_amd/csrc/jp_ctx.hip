@@ -362,6 +362,11 @@ int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value) {
         if (value < 0 || value > 1) return fail(ctx, JP_ERR_ARG, "move phases must be 0 or 1%s");
         ctx->opt_move_phases = (int)value;
         return JP_OK;
+    case JP_OPT_ADVECT_SLOTS:
+        if (value < 0 || value > 1)
+            return fail(ctx, JP_ERR_ARG, "advection slots option must be 0 or 1%s");
+        ctx->opt_advect_slots = (int)value;
+        return JP_OK;
     default:
         return fail(ctx, JP_ERR_ARG, "unknown option%s %lld", "", option);
     }

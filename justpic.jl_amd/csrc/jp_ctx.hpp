@@ -44,6 +44,7 @@ struct jp_ctx {
     // launch options (jp_ctx_set_option); 0 = chosen from the problem size
     int opt_span = 0;
     int opt_move_phases = 0;  // JP_OPT_MOVE_PHASES
+    int opt_advect_slots = 0;  // JP_OPT_ADVECT_SLOTS
     int64_t opt_inject_waves = 0;
     // split geometry: device verdicts (kGeomSlots SplitGeom), the host side
     // of each, the slot of the current call, JP_OPT_GEOMETRY_RECHECK

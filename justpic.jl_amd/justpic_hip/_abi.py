@@ -18,7 +18,8 @@ JP_EULER, JP_RK2, JP_RK4 = 0, 1, 2
 JP_MAX_ARGS = 16
 JP_ENS_VARIANTS = 8
 JP_ADJ_MAX_EVERY = 16
-JP_OPT_SLOT_SPAN, JP_OPT_INJECT_WAVES, JP_OPT_GEOMETRY_RECHECK, JP_OPT_MOVE_PHASES = 1, 2, 3, 4
+JP_OPT_SLOT_SPAN, JP_OPT_INJECT_WAVES, JP_OPT_GEOMETRY_RECHECK = 1, 2, 3
+JP_OPT_MOVE_PHASES, JP_OPT_ADVECT_SLOTS = 4, 5
 JP_NOALLOC_MAX_CELLS = 1 << 21
 JP_MAX_PHASES = 16
 JP_INTERP_LINEAR, JP_INTERP_MQS, JP_INTERP_LINP = 0, 1, 2
