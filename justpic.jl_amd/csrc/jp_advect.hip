@@ -6,8 +6,9 @@
 //
 // The linear scheme (advection!): one workgroup per tile of cells with each
 // velocity component's window of nodes, and its axis windows, staged in LDS;
-// one thread per cell walks the cell's slots (k_advect_tile).  A stage point
-// outside a window sends the particle's whole step to the slot path.  The
+// one thread per cell walks the cell's slots in lockstep chunks of planes
+// (k_advect_tile).  A stage point outside a window sends the particle's whole
+// step to the slot path (inline in 2D, a flagged pass in 3D).  The
 // slot path (k_advect_slot, and the MQS / LinP schemes): one thread per
 // slot, the velocity gathered from global memory.  A bracket lookup costs one
 // guess (uniform spacing estimate) and one strict test against the stored
@@ -368,7 +369,8 @@ __global__ void __launch_bounds__(kBlock) k_advect_slot(Grid<D> g, Vel<D> vg, VP
 
 // The linear scheme with the velocity staged in LDS: one workgroup per tile
 // of cells (2D 64x4, 3D 16x4x4), one thread per cell walking the cell's slot
-// planes (a wavefront covers whole 128-byte runs of a plane).  Each
+// planes in lockstep chunks (a wavefront covers whole 128-byte runs of a
+// plane).  Each
 // component's window holds the nodes from two below the tile's first bracket
 // to kWinPad - 2 beyond its end, which covers the RK stage points of a step of
 // up to one cell; its axis coordinates and inverse spacings are staged too.
@@ -436,11 +438,15 @@ __device__ inline bool interp_tile_f64(const TileLds<D>& L, int c, const double 
     return true;
 }
 
-template <int D, int M>
+// B: slot planes per chunk; FLAGS: particles whose step leaves the windows
+// are flagged (flags[slot], written for every slot of the tile's planes) for
+// k_advect_flagged instead of being advected inline
+template <int D, int M, int B, bool FLAGS>
 __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VPtrs<D> V,
                                                         Ptr3 coords,
                                                         const uint8_t* __restrict__ index,
-                                                        RKParams P, int span) {
+                                                        uint8_t* __restrict__ flags, RKParams P,
+                                                        int span) {
     using TL = F64Tile<D>;
     using WN = F64Win<D>;
     __shared__ TileLds<D> L;
@@ -514,43 +520,89 @@ __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VP
     const int64_t c = lin<D>(g.ni, I);
     const int64_t C = g.C;
     // the step from the windows; if any stage point leaves a window (or sits
-    // on a node), the whole step again on the slot kernel's path
+    // on a node), the whole step again on the slot kernel's path: inline
+    // (2D), or in the flagged pass that follows (3D, where inlining it
+    // doubles the registers)
     bool win = true;
     auto interp = [&](const double (&p)[D], double (&v)[D]) {
 #pragma unroll
         for (int cc = 0; cc < D; ++cc) win = interp_tile_f64<D>(L, cc, p, v[cc]) && win;
     };
-    // the next active slot's position is loaded while this one is advected
     const uint64_t act = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
                                                 : active_slots_in(index, c, C, kb, ke);
-    uint64_t todo = act;
-    double pn[D];
-    int kn = todo ? __builtin_ctzll(todo) : 0;
-    if (todo) {
+    // chunks of B slot planes in lockstep across the wavefront: the chunk's
+    // positions loaded together, its particles advected one at a time, the
+    // results stored plane by plane (each load and store instruction covers
+    // one plane of the wavefront's cells)
+    for (int k0 = kb; k0 < ke; k0 += B) {
+        const uint32_t ab = (uint32_t)(act >> k0) & ((1u << B) - 1);
+        double pb[B][D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) pn[d] = coords.p[d][c + C * kn];
-    }
-    while (todo) {
-        const int k = kn;
-        todo &= todo - 1;
-        double p0[D], out[D];
+        for (int j = 0; j < B; ++j)
 #pragma unroll
-        for (int d = 0; d < D; ++d) p0[d] = pn[d];
-        if (todo) {
-            kn = __builtin_ctzll(todo);
+            for (int d = 0; d < D; ++d) pb[j][d] = ((ab >> j) & 1) ? coords.p[d][c + C * (k0 + j)] : 0.0;
+        uint32_t todo = ab, miss = 0;
+        while (todo) {
+            const int j = __builtin_ctz(todo);
+            todo &= todo - 1;
+            double p0[D], out[D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) pn[d] = coords.p[d][c + C * kn];
+            for (int d = 0; d < D; ++d) {
+                double v = pb[0][d];
+#pragma unroll
+                for (int q = 1; q < B; ++q) v = j == q ? pb[q][d] : v;
+                p0[d] = v;
+            }
+            win = true;
+            advect_rk<D, M>(P, p0, out, interp);
+            if (!win) {
+                if constexpr (FLAGS) {
+                    miss |= 1u << j;  // the flagged pass advects it
+#pragma unroll
+                    for (int d = 0; d < D; ++d) out[d] = p0[d];
+                } else {
+                    AxisGuess<D> Gs;
+                    axis_guess<D>(vg, Gs);
+                    advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < B; ++q)
+#pragma unroll
+                for (int d = 0; d < D; ++d) pb[q][d] = j == q ? out[d] : pb[q][d];
         }
-        win = true;
-        advect_rk<D, M>(P, p0, out, interp);
-        if (!win) {
-            AxisGuess<D> Gs;
-            axis_guess<D>(vg, Gs);
-            advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
-        }
 #pragma unroll
-        for (int d = 0; d < D; ++d) coords.p[d][c + C * k] = out[d];
+        for (int j = 0; j < B; ++j)
+            if ((ab >> j) & 1)
+#pragma unroll
+                for (int d = 0; d < D; ++d) coords.p[d][c + C * (k0 + j)] = pb[j][d];
+        if constexpr (FLAGS) {
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if (k0 + j < ke) flags[c + C * (k0 + j)] = (uint8_t)((miss >> j) & 1);
+        }
     }
+}
+
+// the flagged particles of the tiled kernel, one thread per slot as in
+// k_advect_slot
+template <int D, int M>
+__global__ void __launch_bounds__(kBlock) k_advect_flagged(Grid<D> g, Vel<D> vg, VPtrs<D> V,
+                                                           Ptr3 coords,
+                                                           const uint8_t* __restrict__ flags,
+                                                           RKParams P) {
+    int64_t c, I[D];
+    if (!row_cell<D>(g, c, I)) return;
+    const int64_t e = c + g.C * (int64_t)blockIdx.y;
+    if (!flags[e]) return;
+    AxisGuess<D> Gs;
+    axis_guess<D>(vg, Gs);
+    double p0[D], out[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) p0[d] = coords.p[d][e];
+    advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
+#pragma unroll
+    for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
 }
 
 // semilagrangian_advection! (backtracking.jl:15-101 and the MQS / LinP forms):
@@ -653,8 +705,21 @@ int launch(jp_ctx* ctx, const Grid<D>& G, const Vel<D>& vg, const VPtrs<D>& V, P
             // slot groups when the tiles alone leave the chip under-filled
             const int span = slot_span(ctx, tiles, G.K, 2048);
             const dim3 grid((unsigned)tiles, (unsigned)((G.K + span - 1) / span));
-            k_advect_tile<D, M><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, index, P, span);
-            JP_LAUNCHED(ctx);
+            if constexpr (D == 2) {
+                k_advect_tile<D, M, 8, false><<<grid, kBlock, 0, ctx->stream>>>(
+                    G, vg, V, cp, index, nullptr, P, span);
+                JP_LAUNCHED(ctx);
+            } else {
+                int rc = ensure_scratch(ctx, (size_t)G.C * G.K);
+                if (rc) return rc;
+                uint8_t* flags = static_cast<uint8_t*>(ctx->scratch);
+                k_advect_tile<D, M, 4, true><<<grid, kBlock, 0, ctx->stream>>>(
+                    G, vg, V, cp, index, flags, P, span);
+                JP_LAUNCHED(ctx);
+                const dim3 sgrid((unsigned)row_blocks<D>(G), (unsigned)G.K);
+                k_advect_flagged<D, M><<<sgrid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, flags, P);
+                JP_LAUNCHED(ctx);
+            }
             return JP_OK;
         }
     }
