@@ -422,6 +422,19 @@ int jp_adjoint_forward(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
                        int nsteps, int every, int64_t np, const double* const* x0,
                        const double* Tp, double* ckpt, uint8_t* alive, double* const* xN,
                        double* F, double* W);
+/* jp_adjoint_forward with the trajectories in the FP32-split form: each
+ * particle a cell index plus a float32 offset, every step the split
+ * advection's arithmetic (jp_advection_split) and the split move's
+ * renormalisation, deletion when the cell leaves the grid; checkpoints, xN
+ * and the gather use the FP64 position x_I + u*h, so jp_adjoint_backward
+ * recomputes each block in FP64 from the split trajectory.  The vertex axes
+ * must be uniform and the velocity axes affine (as for the other split entry
+ * points); JP_ERR_ARG otherwise.  No reference counterpart. */
+int jp_adjoint_forward_split(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
+                             const double* const* V, int method, double alpha, double dt,
+                             int nsteps, int every, int64_t np, const double* const* x0,
+                             const double* Tp, double* ckpt, uint8_t* alive, double* const* xN,
+                             double* F, double* W);
 int jp_adjoint_backward(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
                         const double* const* V, int method, double alpha, double dt,
                         int nsteps, int every, int64_t np, const double* ckpt,

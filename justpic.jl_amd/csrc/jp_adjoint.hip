@@ -35,6 +35,7 @@
 // 1/|x - x_node|^2).  Operation order matches the oracle's restatement, so
 // x0bar and Tpbar agree with it bit for bit given the same F, W and Fbar.
 #include "jp_ctx.hpp"
+#include "jp_splitmath.hpp"
 
 #include <cmath>
 
@@ -450,6 +451,113 @@ __global__ __launch_bounds__(kBlock) void k_adj_fwd(AdjArgs<D> A, CPtr3 x0,
     cell[i] = c;
 }
 
+// The forward model in the FP32-split form (jp_adjoint_forward_split): each
+// particle is a cell index plus a float32 offset in cell units, every step
+// the split advection's per-particle arithmetic (jp_splitmath.hpp: velocity
+// scaled to cells per step in float32, every corner read from global memory,
+// the same values a split advection window holds) followed by the split
+// move's renormalisation to the destination cell in FP64; a particle whose
+// cell leaves the grid, or whose offset is not finite, is deleted.  The
+// checkpoints, xN and the gather use the FP64 position x_I + u*h, so the
+// FP64 backward recomputes each block from the split trajectory.
+struct NoWindow {  // every corner from global memory
+    __host__ __device__ static constexpr int n(int) { return 1; }
+    __host__ __device__ static constexpr int w(int) { return 0; }
+    __host__ __device__ static constexpr int size() { return 1; }
+};
+
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_adj_fwd_split(
+    AdjArgs<D> A, split::SVelIn<D> SV, const SplitGeom* __restrict__ geo, split::RKf P, CPtr3 x0,
+    double* __restrict__ ckpt, uint8_t* __restrict__ alive, Ptr3 xN, int32_t* __restrict__ cell,
+    int32_t* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np || geo->bad) return;  // a rejected grid is reported by the call
+    split::SMap<D> m;
+    double sc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        sc[c] = A.dt / A.G.xv[c].dx[0];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            m.r[c][d] = geo->r[c][d];
+            m.o[c][d] = geo->o[c][d];
+        }
+    }
+    double x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = x0.p[d][i];
+    bool a = indomain(A.G, x);
+    int I[D];
+    float u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I[d] = 0;
+        u[d] = 0.0f;
+        if (a) {
+            const Axis& X = A.G.xv[d];
+            I[d] = (int)adj_bracket(x[d], X);
+            const double ud = (x[d] - X.x[I[d]]) * X.idx[I[d]];
+            float uf = (float)ud;
+            if (ud < 1.0 && uf >= 1.0f) uf = 0x1.fffffep-1f;  // stays inside its cell
+            u[d] = uf;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) ckpt[(int64_t)d * A.np + i] = x[d];
+    const int w0[D][D] = {};
+    int64_t blk = 0;
+    int left = A.every;
+    for (int s = 0; s < A.nsteps; ++s) {
+        if (left == 0) {
+            ++blk;
+            left = A.every;
+#pragma unroll
+            for (int d = 0; d < D; ++d) ckpt[(blk * D + d) * A.np + i] = x[d];
+        }
+        --left;
+        if (!a) continue;
+        split::CellMap<D> cm;
+        split::cell_map<D, false>(m, I, cm);
+        float out[D];
+        split::advect_split_one<D, M, NoWindow>(nullptr, SV, sc, w0, I, cm, P, u, out);
+        bool gone = false;
+        int sd[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const bool fin = fabsf(out[d]) < 1.0e6f;  // false for NaN/Inf
+            sd[d] = fin ? (int)floorf(out[d]) : 0;
+            const int J = I[d] + sd[d];
+            gone = gone || !fin || J < 0 || J >= A.G.ni[d];
+        }
+        if (gone) {  // deleted: the last position, as the FP64 model keeps it
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                x[d] = A.G.xv[d].x[I[d]] + (double)out[d] * A.G.xv[d].dx[I[d]];
+            a = false;
+            continue;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            I[d] += sd[d];
+            u[d] = (float)((double)out[d] - (double)sd[d]);
+            x[d] = A.G.xv[d].x[I[d]] + (double)u[d] * A.G.xv[d].dx[I[d]];
+        }
+    }
+    alive[i] = a ? 1 : 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) xN.p[d][i] = x[d];
+    int32_t c = -1;
+    if (a) {
+        int64_t k[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) k[d] = I[d];
+        c = (int32_t)lin<D>(A.G.ni, k);
+        atomicAdd(cnt + c, 1);
+    }
+    cell[i] = c;
+}
+
 // exclusive scan of the C cell counts into start[0..C] (one workgroup)
 constexpr int kScan = 1024;
 __global__ __launch_bounds__(kScan) void k_adj_scan(const int32_t* __restrict__ cnt,
@@ -810,10 +918,20 @@ int64_t node_count(const jp_cellgrid* g) {
     return n;
 }
 
+// the trajectories (FP64, or the split form with its velocity description,
+// RK parameters and the current geometry verdict), then the gather
+template <int D>
+struct SplitFwd {
+    bool on = false;
+    split::SVelIn<D> SV;
+    split::RKf P;
+    const SplitGeom* geo = nullptr;
+};
+
 template <int D, int M>
 int forward(jp_ctx* ctx, const AdjArgs<D>& A, const double* const* x0, const double* Tp,
             double* ckpt, uint8_t* alive, double* const* xN, double* F, double* W,
-            int64_t nodes) {
+            int64_t nodes, const SplitFwd<D>& sf = SplitFwd<D>()) {
     // scratch: counts (C) | bucket starts (C+1) | cell of each particle | buckets
     const int64_t C = A.G.C;
     const size_t a = ((size_t)C * 4 + 255) & ~(size_t)255;
@@ -835,8 +953,12 @@ int forward(jp_ctx* ctx, const AdjArgs<D>& A, const double* const* x0, const dou
         xq.p[d] = xN[d];
     }
     if (A.np > 0) {
-        k_adj_fwd<D, M><<<(unsigned)blocks_for(A.np), kBlock, 0, ctx->stream>>>(
-            A, xp, ckpt, alive, xo, cell, cnt);
+        if (sf.on)
+            k_adj_fwd_split<D, M><<<(unsigned)blocks_for(A.np), kBlock, 0, ctx->stream>>>(
+                A, sf.SV, sf.geo, sf.P, xp, ckpt, alive, xo, cell, cnt);
+        else
+            k_adj_fwd<D, M><<<(unsigned)blocks_for(A.np), kBlock, 0, ctx->stream>>>(
+                A, xp, ckpt, alive, xo, cell, cnt);
         JP_LAUNCHED(ctx);
     }
     k_adj_scan<<<1, kScan, 0, ctx->stream>>>(cnt, start, C);
@@ -851,7 +973,7 @@ int forward(jp_ctx* ctx, const AdjArgs<D>& A, const double* const* x0, const dou
     k_adj_gather<D><<<(unsigned)blocks_for(nodes), kBlock, 0, ctx->stream>>>(A, start, list, xq,
                                                                             Tp, F, W, nodes);
     JP_LAUNCHED(ctx);
-    return finish(ctx);
+    return sf.on ? split_finish(ctx) : finish(ctx);
 }
 
 template <int D, int M>
@@ -919,6 +1041,56 @@ extern "C" int jp_adjoint_forward(jp_ctx* ctx, const jp_cellgrid* g, const jp_ve
         return forward<3, JP_EULER>(ctx, A, x0, Tp, ckpt, alive, xN, F, W, nodes);
     if (method == JP_RK2) return forward<3, JP_RK2>(ctx, A, x0, Tp, ckpt, alive, xN, F, W, nodes);
     return forward<3, JP_RK4>(ctx, A, x0, Tp, ckpt, alive, xN, F, W, nodes);
+}
+
+namespace {
+
+template <int D>
+int forward_split(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
+                  const double* const* V, int method, double alpha, double dt, int nsteps,
+                  int every, int64_t np, const double* const* x0, const double* Tp, double* ckpt,
+                  uint8_t* alive, double* const* xN, double* F, double* W) {
+    const auto A = make_args<D>(g, vg, V, alpha, dt, nsteps, every, np);
+    int rc = split_geometry(ctx, g, vg);
+    if (rc) return rc;
+    SplitFwd<D> sf;
+    sf.on = true;
+    sf.geo = geo_ptr(ctx);
+    for (int c = 0; c < D; ++c) {
+        sf.SV.v[c] = V[c];
+        for (int d = 0; d < D; ++d) sf.SV.n[c][d] = (int)vg->ax[c][d].n;
+    }
+    sf.P.alpha = (float)alpha;
+    sf.P.half = alpha == 0.5;
+    const double ia = 1.0 / alpha;
+    sf.P.w0 = (float)std::fma(-0.5, ia, 1.0);
+    sf.P.w1 = (float)(0.5 * ia);
+    const int64_t nodes = node_count<D>(g);
+    if (method == JP_EULER)
+        return forward<D, JP_EULER>(ctx, A, x0, Tp, ckpt, alive, xN, F, W, nodes, sf);
+    if (method == JP_RK2)
+        return forward<D, JP_RK2>(ctx, A, x0, Tp, ckpt, alive, xN, F, W, nodes, sf);
+    return forward<D, JP_RK4>(ctx, A, x0, Tp, ckpt, alive, xN, F, W, nodes, sf);
+}
+
+}  // namespace
+
+extern "C" int jp_adjoint_forward_split(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,
+                                        const double* const* V, int method, double alpha,
+                                        double dt, int nsteps, int every, int64_t np,
+                                        const double* const* x0, const double* Tp, double* ckpt,
+                                        uint8_t* alive, double* const* xN, double* F, double* W) {
+    int rc = check_args(ctx, g, vg, V, method, alpha, nsteps, every, np);
+    if (rc) return rc;
+    if (!F || !W) return fail(ctx, JP_ERR_ARG, "null nodal output%s");
+    if ((rc = null_ptrs(ctx, g->ndim, np, (const void* const*)x0, g->ndim, "x0"))) return rc;
+    if ((rc = null_ptrs(ctx, g->ndim, np, (const void* const*)xN, g->ndim, "xN"))) return rc;
+    if (np > 0 && (!Tp || !ckpt || !alive))
+        return fail(ctx, JP_ERR_ARG, "null particle array (Tp, ckpt or alive)%s");
+    return g->ndim == 2 ? forward_split<2>(ctx, g, vg, V, method, alpha, dt, nsteps, every, np,
+                                           x0, Tp, ckpt, alive, xN, F, W)
+                        : forward_split<3>(ctx, g, vg, V, method, alpha, dt, nsteps, every, np,
+                                           x0, Tp, ckpt, alive, xN, F, W);
 }
 
 extern "C" int jp_adjoint_backward(jp_ctx* ctx, const jp_cellgrid* g, const jp_velgrid* vg,

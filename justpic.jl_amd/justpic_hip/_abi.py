@@ -154,6 +154,9 @@ _SIGS = {
     "jp_ensemble_destroy": (C.c_int, [_P, _P]),
     "jp_adjoint_forward": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
                                      C.c_int, C.c_int64, _P, _P, _P, _P, _P, _P, _P]),
+    "jp_adjoint_forward_split": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
+                                           C.c_int, C.c_int, C.c_int64, _P, _P, _P, _P, _P, _P,
+                                           _P]),
     "jp_adjoint_backward": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P,
                                       _P]),

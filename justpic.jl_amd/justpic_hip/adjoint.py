@@ -56,12 +56,21 @@ def _as_device(ctx, a, shape=None):
 
 class AdvectionP2G:
     """``F = particle2grid(advect^nsteps(x0; V), Tp)`` on the grids of
-    ``particles`` (its vertex axes and staggered velocity axes)."""
+    ``particles`` (its vertex axes and staggered velocity axes).
+
+    ``form``: "f64" runs the trajectories in FP64 (``jp_adjoint_forward``),
+    "split" in the FP32-split form (``jp_adjoint_forward_split``: a cell index
+    and a float32 offset per particle, the split advection's arithmetic; the
+    checkpoints hold the FP64 positions of the split trajectory, from which
+    the FP64 pullback recomputes each block)."""
 
     def __init__(self, particles: Particles, method: AbstractAdvectionIntegrator | None,
-                 dt: float, nsteps: int, every: int = 10):
+                 dt: float, nsteps: int, every: int = 10, form: str = "f64"):
         if getattr(particles, "form", "f64") != "f64":
             raise ValueError("the adjoint runs on FP64 grids (an FP64 container)")
+        if form not in ("f64", "split"):
+            raise ValueError("form must be 'f64' or 'split'")
+        self.form = form
         if not 1 <= int(every) <= _abi.JP_ADJ_MAX_EVERY:
             raise ValueError(f"every must be in 1..{_abi.JP_ADJ_MAX_EVERY}")
         if int(nsteps) < 0:
@@ -103,7 +112,8 @@ class AdvectionP2G:
         else:  # the same buffers: a new forward pass overwrites the previous one
             fw = self._fw
             ckpt, alive, xN, F, W = fw["ckpt"], fw["alive"], fw["xN"], fw["F"], fw["W"]
-        ctx.check(ctx.lib.jp_adjoint_forward(
+        entry = ctx.lib.jp_adjoint_forward if self.form == "f64" else ctx.lib.jp_adjoint_forward_split
+        ctx.check(entry(
             ctx.h, self.p.grid_ref(), self.p.velgrid_ref(), _abi.ptr_array([v.ptr for v in V]),
             self.method.code, self.method.alpha, self.dt, self.nsteps, self.every, n,
             _abi.ptr_array([x.ptr for x in x0]), Tp.ptr, ckpt.ptr, alive.ptr,
@@ -153,10 +163,11 @@ class AdvectionP2G:
         return Vbar, x0bar, Tpbar
 
 
-def rrule_advection_p2g(particles: Particles, method, V, dt, nsteps, x0, Tp, every: int = 10):
+def rrule_advection_p2g(particles: Particles, method, V, dt, nsteps, x0, Tp, every: int = 10,
+                        form: str = "f64"):
     """ChainRulesCore-style ``rrule``: ``(F, pullback)`` with
-    ``pullback(Fbar) -> (Vbar, x0bar, Tpbar)``."""
-    op = AdvectionP2G(particles, method, dt, nsteps, every)
+    ``pullback(Fbar) -> (Vbar, x0bar, Tpbar)``; ``form`` as in AdvectionP2G."""
+    op = AdvectionP2G(particles, method, dt, nsteps, every, form)
     F = op.forward(V, x0, Tp)
     return F, op.pullback
 
