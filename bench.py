@@ -108,8 +108,9 @@ def cpu_baseline(cfg, spec, n_steps, seed):
 
 CALLS = ("advection", "move", "inject", "p2g")
 # the kernels each call launches
-KERNELS = {"advection": "one advection kernel",
-           "move": "classify, 3^D colour phases, scatter, serial fallback",
+KERNELS = {"advection": "one advection kernel (3D FP64: plus the flagged slot-path pass)",
+           "move": "classify, colour phases (2D: one tiled launch; 3D: 3^D launches), "
+                   "far-jump check, scatter",
            "inject": "scan, list join, 2^D colour wavefront kernels",
            "p2g": "per-cell partial sums and per-node gather (split) or occupancy words "
                   "and per-node gather (FP64)"}
@@ -412,6 +413,43 @@ def run_adjoint(jp, args):
             bw_ms.append(ctx.elapsed_ms(e[2], e[3]))
             losses.append(L)
     fw, bw = float(np.mean(fw_ms)), float(np.mean(bw_ms))
+    # the forward pass in the FP32-split form beside it: time, and its F,
+    # final positions and gradient against the FP64 forward's
+    F64 = op.forward(Vg, x0d, Tp).flat()
+    xN64 = [x.flat()[:P] for x in op.xN]
+    al64 = op.alive.flat()[:P].astype(bool)
+    L64, fb64 = jp.l2_loss(op._fw["F"], Ft)
+    Fb.copy_from(fb64)
+    g64 = [v.to_host().ravel(order="F") for v in op.pullback(Fb)[0]]
+    ops = jp.AdvectionP2G(p, rk2, cfg["dt"], N, every, form="split")
+    sw_ms = []
+    for it in range(args.warmup + args.steps):
+        ctx.record(e[0])
+        Fs = ops.forward(Vg, x0d, Tp)
+        ctx.record(e[1])
+        ctx.sync()
+        if it >= args.warmup:
+            sw_ms.append(ctx.elapsed_ms(e[0], e[1]))
+    Fs = Fs.flat()
+    als = ops.alive.flat()[:P].astype(bool)
+    both = als & al64
+    xerr = max(float(np.abs(x.flat()[:P][both] - y[both]).max()) for x, y in zip(ops.xN, xN64))
+    okF = np.isfinite(Fs) & np.isfinite(F64)
+    Ls, fbs = jp.l2_loss(ops._fw["F"], Ft)
+    Fb.copy_from(fbs)
+    gs = [v.to_host().ravel(order="F") for v in ops.pullback(Fb)[0]]
+    grel = np.concatenate([np.abs(a[np.abs(b) >= 1e-3 * np.abs(b).max()] -
+                                  b[np.abs(b) >= 1e-3 * np.abs(b).max()]) /
+                           np.abs(b[np.abs(b) >= 1e-3 * np.abs(b).max()])
+                           for a, b in zip(gs, g64)])
+    split = {"forward_ms": float(np.mean(sw_ms)), "loss": Ls, "loss_fp64": L64,
+             "xN_max_abs_err_vs_fp64": xerr,
+             "survivor_flips": int(np.count_nonzero(als != al64)),
+             "F_max_abs_diff_vs_fp64": float(np.abs(Fs[okF] - F64[okF]).max()),
+             "grad_rel_diff_vs_fp64": {"median": float(np.median(grel)),
+                                       "p99": float(np.quantile(grel, 0.99)),
+                                       "max": float(grel.max()), "entries": int(grel.size)}}
+    log("split forward", split)
     # inverse problem: a few steepest-descent steps with backtracking
     V = [v.to_host() for v in Vg]
     L0 = losses[0]
@@ -476,7 +514,8 @@ def run_adjoint(jp, args):
                    "particles": P, "forward_ms": fw, "backward_ms": bw,
                    "loss_history": hist, "max_abs_V_error_vs_true": err_true,
                    "initial_max_abs_V_error": max(float(np.abs(a - b).max()) for a, b in
-                                                  zip(cfg["V"], cfg["V_true"]))},
+                                                  zip(cfg["V"], cfg["V_true"])),
+                   "split_forward": split},
         "roofline": {"bound": "hbm", "kernel": "adjoint forward pass (all launches)",
                      "achieved": fw_bytes / (fw * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": fw_bytes / (fw * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -516,8 +555,9 @@ def main():
                     help="integrator (default: RK2 for step, RK4 for step3d)")
     ap.add_argument("--no-targets", action="store_true",
                     help="skip the target rows (split vs FP64 RK2 at 256^2 and 128^3)")
-    ap.add_argument("--cpu-steps", type=int, default=6,
-                    help="oracle steps timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="oracle steps timed for the CPU baseline, about 10 s on 16 host "
+                         "threads at 512^2 (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "

@@ -505,19 +505,21 @@ __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VP
             }
     }
     __syncthreads();
-    int64_t I[D];
-    bool inside = true;
-    {
-        int r = tid;
+    // the cell of thread t of the tile and whether it is in the grid
+    auto cell_of = [&](int t, int64_t (&J)[D]) {
+        bool in = true;
+        int r = t;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            I[d] = t0[d] + r % TL::n[d];
+            J[d] = t0[d] + r % TL::n[d];
             r /= TL::n[d];
-            inside = inside && I[d] < g.ni[d];
+            in = in && J[d] < g.ni[d];
         }
-    }
-    if (!inside) return;
-    const int64_t c = lin<D>(g.ni, I);
+        return in;
+    };
+    int64_t I[D];
+    const bool inside = cell_of(tid, I);
+    const int64_t c = inside ? lin<D>(g.ni, I) : 0;
     const int64_t C = g.C;
     // the step from the windows; if any stage point leaves a window (or sits
     // on a node), the whole step again on the slot kernel's path: inline
@@ -528,58 +530,70 @@ __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VP
 #pragma unroll
         for (int cc = 0; cc < D; ++cc) win = interp_tile_f64<D>(L, cc, p, v[cc]) && win;
     };
-    const uint64_t act = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
-                                                : active_slots_in(index, c, C, kb, ke);
-    // chunks of B slot planes in lockstep across the wavefront: the chunk's
-    // positions loaded together, its particles advected one at a time, the
-    // results stored plane by plane (each load and store instruction covers
-    // one plane of the wavefront's cells)
+    // a thread outside the grid owns no particle but still takes work
+    const uint64_t act = !inside ? 0ull
+                                 : ((kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                           : active_slots_in(index, c, C, kb, ke));
+    // Chunks of B slot planes.  The wavefront's active particles of a chunk
+    // are listed (plane-major, then lane) and dealt to its lanes in turn, so
+    // that every lane advects about the same number of particles; consecutive
+    // entries are neighbouring cells of one plane, so the loads and stores
+    // stay coalesced.  The list is written and read by the same wavefront
+    // (its LDS operations complete in order).
+    __shared__ uint16_t s_items[kBlock / 64][B * 64];
+    uint16_t* items = s_items[tid >> 6];
+    const int lane = tid & 63, wbase = tid & ~63;
     for (int k0 = kb; k0 < ke; k0 += B) {
-        const uint32_t ab = (uint32_t)(act >> k0) & ((1u << B) - 1);
-        double pb[B][D];
+        if constexpr (FLAGS) {  // every slot of the chunk: no miss yet
 #pragma unroll
-        for (int j = 0; j < B; ++j)
+            for (int j = 0; j < B; ++j)
+                if (inside && k0 + j < ke) flags[c + C * (k0 + j)] = 0;
+        }
+        int n = 0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) pb[j][d] = ((ab >> j) & 1) ? coords.p[d][c + C * (k0 + j)] : 0.0;
-        uint32_t todo = ab, miss = 0;
-        while (todo) {
-            const int j = __builtin_ctz(todo);
-            todo &= todo - 1;
-            double p0[D], out[D];
+        for (int j = 0; j < B; ++j) {
+            const bool a = k0 + j < 64 && ((act >> (k0 + j)) & 1);
+            const uint64_t bal = __ballot(a);
+            const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (a) items[n + below] = (uint16_t)((j << 6) | lane);
+            n += __popcll(bal);
+        }
+        if (!n) continue;
+        // this lane's entries lane, lane + 64, ...: positions loaded first
+        double pq[B][D];
+        int dq[B];
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                double v = pb[0][d];
+        for (int q = 0; q < B; ++q) {
+            const int item = lane + 64 * q;
+            dq[q] = item < n ? items[item] : -1;
+            int64_t J[D];
+            cell_of(wbase + (dq[q] & 63), J);
+            const int64_t e = lin<D>(g.ni, J) + C * (k0 + (dq[q] >> 6));
 #pragma unroll
-                for (int q = 1; q < B; ++q) v = j == q ? pb[q][d] : v;
-                p0[d] = v;
-            }
+            for (int d = 0; d < D; ++d) pq[q][d] = dq[q] >= 0 ? coords.p[d][e] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            if (dq[q] < 0) break;
+            int64_t J[D];
+            cell_of(wbase + (dq[q] & 63), J);
+            const int64_t e = lin<D>(g.ni, J) + C * (k0 + (dq[q] >> 6));
+            double out[D];
             win = true;
-            advect_rk<D, M>(P, p0, out, interp);
+            advect_rk<D, M>(P, pq[q], out, interp);
             if (!win) {
                 if constexpr (FLAGS) {
-                    miss |= 1u << j;  // the flagged pass advects it
-#pragma unroll
-                    for (int d = 0; d < D; ++d) out[d] = p0[d];
+                    flags[e] = 1;  // the flagged pass advects it
+                    continue;
                 } else {
                     AxisGuess<D> Gs;
                     axis_guess<D>(vg, Gs);
-                    advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
+                    advect_slot<D, M>(Gs, vg, V, P, pq[q], J, out);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < B; ++q)
-#pragma unroll
-                for (int d = 0; d < D; ++d) pb[q][d] = j == q ? out[d] : pb[q][d];
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j)
-            if ((ab >> j) & 1)
-#pragma unroll
-                for (int d = 0; d < D; ++d) coords.p[d][c + C * (k0 + j)] = pb[j][d];
-        if constexpr (FLAGS) {
-#pragma unroll
-            for (int j = 0; j < B; ++j)
-                if (k0 + j < ke) flags[c + C * (k0 + j)] = (uint8_t)((miss >> j) & 1);
+            for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
         }
     }
 }

@@ -215,63 +215,82 @@ __device__ inline void adv_split_tile(const SGrid<D>& g, const SVelIn<D>& V, con
         for (int it = 0; it < NI; ++it)
             if (tid + it * kBlock < TS::size()) sV[c][tid + it * kBlock] = wv[c][it];
     __syncthreads();
-    int I[D];
-    bool in = true;
-    {
-        int r = tid;
+    // the cell of thread t of the tile, its linear index, and whether it is
+    // in the grid
+    auto cell_of = [&](int t, int (&J)[D], int64_t& cj) {
+        bool in = true;
+        int r = t;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            I[d] = t0[d] + r % TS::n(d);
+            J[d] = t0[d] + r % TS::n(d);
             r /= TS::n(d);
-            in = in && I[d] < g.ni[d];
+            in = in && J[d] < g.ni[d];
         }
-    }
-    if (!in) return;
-    CellMap<D> cm;
-    cell_map<D, STD>(m, I, cm);
-    int64_t c = I[D - 1];
+        cj = J[D - 1];
 #pragma unroll
-    for (int d = D - 2; d >= 0; --d) c = c * g.ni[d] + I[d];
+        for (int d = D - 2; d >= 0; --d) cj = cj * g.ni[d] + J[d];
+        return in;
+    };
+    int I[D];
+    int64_t c;
+    const bool in = cell_of(tid, I, c);
     const int64_t C = g.C;
-    const uint64_t act = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
-                                                : active_slots_in(index, c, C, kb, ke);
-    constexpr int B = 8;
-    for (int k0 = kb; k0 < ke; k0 += B) {
-        const uint32_t ab = (uint32_t)(act >> k0) & 0xFFu;
-        if (!ab) continue;
-        float ub[B][D];
+    // a thread outside the grid owns no particle but still takes work
+    const uint64_t act = !in ? 0ull
+                             : ((kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                       : active_slots_in(index, c, C, kb, ke));
+    // Chunks of CH slot planes.  The wavefront's active particles of a chunk
+    // are listed (plane-major, then lane) and dealt to its lanes in turn, so
+    // that every lane advects about the same number of particles (a lane per
+    // cell walking its own slots idles whenever its cell has fewer active
+    // slots in the chunk than the fullest cell); consecutive list entries are
+    // neighbouring cells of one plane, so the loads and stores stay
+    // coalesced.  The list is written and read by the same wavefront (its LDS
+    // operations complete in order).
+    constexpr int CH = D == 2 ? 8 : 4;  // 3D: fewer positions in registers
+    __shared__ uint16_t s_items[kBlock / 64][CH * 64];
+    uint16_t* items = s_items[tid >> 6];
+    const int lane = tid & 63, wbase = tid & ~63;
+    for (int k0 = kb; k0 < ke; k0 += CH) {
+        int n = 0;
 #pragma unroll
-        for (int j = 0; j < B; ++j)
+        for (int j = 0; j < CH; ++j) {
+            const bool a = k0 + j < 64 && ((act >> (k0 + j)) & 1);
+            const uint64_t bal = __ballot(a);
+            const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (a) items[n + below] = (uint16_t)((j << 6) | lane);
+            n += __popcll(bal);
+        }
+        if (!n) continue;
+        // this lane's entries lane, lane + 64, ...: positions loaded first
+        float uq[CH][D];
+        int dq[CH];
 #pragma unroll
-            for (int d = 0; d < D; ++d) ub[j][d] = ((ab >> j) & 1) ? off.p[d][c + C * (k0 + j)] : 0.0f;
-        // one particle at a time (the chunk's loads are already in flight);
-        // the results go back into ub and are stored plane by plane after the
-        // chunk, so that each store instruction covers one slot plane of the
-        // wavefront's cells (a store per particle would scatter a wavefront's
-        // lanes over the chunk's planes)
-        uint32_t todo = ab;
-        while (todo) {
-            const int j = __builtin_ctz(todo);
-            todo &= todo - 1;
-            float u0[D], out[D];
+        for (int q = 0; q < CH; ++q) {
+            const int item = lane + 64 * q;
+            dq[q] = item < n ? items[item] : -1;
+            int J[D];
+            int64_t cj;
+            cell_of(wbase + (dq[q] & 63), J, cj);
+            const int64_t e = cj + C * (k0 + (dq[q] >> 6));
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                float v = ub[0][d];
-#pragma unroll
-                for (int q = 1; q < B; ++q) v = j == q ? ub[q][d] : v;
-                u0[d] = v;
-            }
-            advect_split_one<D, M, TS>(sV, V, sc, w0, I, cm, P, u0, out);
-#pragma unroll
-            for (int q = 0; q < B; ++q)
-#pragma unroll
-                for (int d = 0; d < D; ++d) ub[q][d] = j == q ? out[d] : ub[q][d];
+            for (int d = 0; d < D; ++d) uq[q][d] = dq[q] >= 0 ? off.p[d][e] : 0.0f;
         }
 #pragma unroll
-        for (int j = 0; j < B; ++j)
-            if ((ab >> j) & 1)
+        for (int q = 0; q < CH; ++q) {
+            if (dq[q] < 0) break;
+            int J[D];
+            int64_t cj;
+            cell_of(wbase + (dq[q] & 63), J, cj);
+            CellMap<D> cm;
+            cell_map<D, STD>(m, J, cm);
+            float out[D];
+            advect_split_one<D, M, TS>(sV, V, sc, w0, J, cm, P, uq[q], out);
+            const int64_t e = cj + C * (k0 + (dq[q] >> 6));
 #pragma unroll
-                for (int d = 0; d < D; ++d) off.p[d][c + C * (k0 + j)] = ub[j][d];
+            for (int d = 0; d < D; ++d) off.p[d][e] = out[d];
+        }
     }
 }
 
