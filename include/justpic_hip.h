@@ -98,8 +98,15 @@ void* jp_ctx_stream(jp_ctx* ctx);
  *                       advection with the linear interpolation): 0
  *                       (default) chooses from the grid size; otherwise a
  *                       multiple of 8 up to 64 (>= max_xcell: whole cells).
- *   JP_OPT_INJECT_WAVES persistent wavefronts per injection colour launch: 0
- *                       (default) is one per listed cell up to 4096.
+ *   JP_OPT_INJECT_WAVES persistent wavefronts per injection launch: 0
+ *                       (default) is one per listed cell up to 4096 (the
+ *                       ordered 2D launch is also capped at the number the
+ *                       device holds at once).
+ *   JP_OPT_INJECT_PHASES the 2D injection (at most one field): 0 (default)
+ *                       the four colours in one launch, each listed cell
+ *                       waiting for its listed neighbours of earlier
+ *                       colours; 1 one launch per colour (the form the 3D
+ *                       injection always takes).
  *   JP_OPT_GEOMETRY_RECHECK  the FP32-split axis check: 0 (default) once per
  *                       set of axes (pointers and lengths) until any jp_free,
  *                       1 on every call (for axes whose coordinates the
@@ -118,6 +125,7 @@ void* jp_ctx_stream(jp_ctx* ctx);
 #define JP_OPT_GEOMETRY_RECHECK 3
 #define JP_OPT_MOVE_PHASES 4
 #define JP_OPT_ADVECT_SLOTS 5
+#define JP_OPT_INJECT_PHASES 6
 int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value);
 int jp_sync(jp_ctx* ctx);
 int jp_malloc(jp_ctx* ctx, void** dptr, size_t bytes);

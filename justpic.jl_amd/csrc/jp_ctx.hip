@@ -188,15 +188,16 @@ int occ_words(jp_ctx* ctx, const uint8_t* index, int64_t C, int K, uint64_t* occ
 }
 
 int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, InjectLists& L) {
-    // scratch: totals | counts | segments | lists | occupancy words | boxes
+    // scratch: totals | counts | segments | lists | occupancy words | boxes | states
     L.nwave = (C + 63) / 64;
     const size_t a = 256;
     const size_t b = a + (((size_t)nq * L.nwave * 4 + 255) & ~(size_t)255);
     const size_t c = b + (size_t)nq * L.nwave * 64 * 4;
     const size_t d = c + (((size_t)nq * cap * 4 + 255) & ~(size_t)255);
     const size_t e = d + (((size_t)C * 8 + 255) & ~(size_t)255);
-    const size_t f = e + box_bytes;
-    int rc = ensure_scratch(ctx, f);
+    const size_t f = e + ((box_bytes + 255) & ~(size_t)255);
+    const size_t h = f + (size_t)C * 4;
+    int rc = ensure_scratch(ctx, h);
     if (rc) return rc;
     char* base = static_cast<char*>(ctx->scratch);
     L.total = reinterpret_cast<int32_t*>(base);
@@ -205,6 +206,7 @@ int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, 
     L.list = reinterpret_cast<int32_t*>(base + c);
     L.occ = reinterpret_cast<uint64_t*>(base + d);
     L.box = box_bytes ? static_cast<void*>(base + e) : nullptr;
+    L.state = reinterpret_cast<uint32_t*>(base + f);
     L.cap = cap;
     return JP_OK;
 }
@@ -245,6 +247,17 @@ int stats_end(jp_ctx* ctx, jp_stats* host) {
     host->injected += tmp.injected;
     host->orphan += tmp.orphan;
     return JP_OK;
+}
+
+int inject_wait_report(jp_ctx* ctx) {
+    int v = 0;
+    JP_HIP(ctx, hipMemcpyAsync(&v, dinject(ctx), sizeof v, hipMemcpyDeviceToHost, ctx->stream));
+    JP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!v) return JP_OK;
+    int rc = zero_bytes(ctx, dinject(ctx), sizeof v);
+    if (rc) return rc;
+    return fail(ctx, JP_ERR_STATE,
+                "inject_particles!: a cell's wait for an earlier-colour neighbour timed out%s");
 }
 
 int finish(jp_ctx* ctx) {
@@ -357,6 +370,10 @@ int jp_ctx_set_option(jp_ctx* ctx, int option, int64_t value) {
         if (value < 0 || value > 1)
             return fail(ctx, JP_ERR_ARG, "geometry recheck must be 0 or 1%s");
         ctx->opt_geom_recheck = (int)value;
+        return JP_OK;
+    case JP_OPT_INJECT_PHASES:
+        if (value < 0 || value > 1) return fail(ctx, JP_ERR_ARG, "inject phases must be 0 or 1%s");
+        ctx->opt_inject_phases = (int)value;
         return JP_OK;
     case JP_OPT_MOVE_PHASES:
         if (value < 0 || value > 1) return fail(ctx, JP_ERR_ARG, "move phases must be 0 or 1%s");

@@ -46,6 +46,7 @@ struct jp_ctx {
     int opt_move_phases = 0;  // JP_OPT_MOVE_PHASES
     int opt_advect_slots = 0;  // JP_OPT_ADVECT_SLOTS
     int64_t opt_inject_waves = 0;
+    int opt_inject_phases = 0;  // JP_OPT_INJECT_PHASES
     // split geometry: device verdicts (kGeomSlots SplitGeom), the host side
     // of each, the slot of the current call, JP_OPT_GEOMETRY_RECHECK
     void* geo_dev = nullptr;
@@ -107,6 +108,12 @@ inline int* drange(jp_ctx* ctx) {
 }
 // report (and clear) an out-of-grid backtrack as JP_ERR_ARG
 int range_report(jp_ctx* ctx);
+// set by the ordered 2D injection when a dependency wait gave up
+inline int* dinject(jp_ctx* ctx) {
+    return reinterpret_cast<int*>(static_cast<char*>(ctx->dsmall) + 3584);
+}
+// report (and clear) such a timeout as JP_ERR_STATE (synchronises)
+int inject_wait_report(jp_ctx* ctx);
 // report one left by asynchronous semilagrangian calls
 int range_sticky(jp_ctx* ctx);
 
@@ -138,6 +145,7 @@ struct InjectLists {
     int64_t cap;
     uint64_t* occ;  // occupancy word of every cell
     void* box;      // 2*D planes of C: per-cell min, max of each coordinate (2D)
+    uint32_t* state;  // per cell, written by the scan: 1 listed, 0 not (ordered 2D injection)
 };
 int inject_lists(jp_ctx* ctx, int64_t C, int nq, int64_t cap, size_t box_bytes, InjectLists& L);
 int join_segments(jp_ctx* ctx, int nq, const InjectLists& L);

@@ -297,6 +297,7 @@ __global__ void __launch_bounds__(kBlock) k_inject_scan(Grid<D> g, CPtr3 coords,
 #pragma unroll
         for (int q = 1; q < (1 << D); ++q) want = want || (valid && cnt_q[q] < min_q);
     }
+    if (valid && L.state) L.state[c] = want ? 1u : 0u;
     int col = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) col = col * 2 + (int)(I[d] & 1);
@@ -658,6 +659,9 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
     if ((rc = inject_lists(ctx, G.C, NQ, cap, 0, L))) return rc;
+    const bool two = D == 2 && nargs <= 1;  // the 2D kernel (k_inject2)
+    const bool ordered = two && inject2_ordered(ctx);
+    if (!ordered) L.state = nullptr;
     k_inject_scan<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D), index,
                                                                   min_q, L);
     JP_LAUNCHED(ctx);
@@ -665,35 +669,32 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, double* const* coords, uint8_
     Ptr3 cp = ptr3(coords, D);
     ArgPtrs ap = argptrs(args, nargs);
     jp_stats* st = stats ? dstats(ctx) : nullptr;
-    for (int ph = 0; ph < NQ; ++ph) {
-        // colour index of the phase: offsets (o_1 outer ... o_D inner)
-        // persistent wavefronts, one listed cell at a time each
-        if constexpr (D == 2) {
-            if (nargs <= 1) {
-                Inject2 a{};
-                a.g = G;
-                a.x[0] = coords[0];
-                a.x[1] = coords[1];
-                a.index = index;
-                a.arg = nargs ? args[0] : nullptr;
-                a.min_q = min_q;
-                a.seed = seed;
-                a.counter = counter;
-                a.list = L.list + (int64_t)ph * cap;
-                a.count = L.total + ph;
-                a.occw = L.occ;
-                a.st = st;
-                k_inject2<false><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(a);
-                JP_LAUNCHED(ctx);
-                continue;
-            }
+    if constexpr (D == 2) {
+        if (two) {
+            Inject2 a{};
+            a.g = G;
+            a.x[0] = coords[0];
+            a.x[1] = coords[1];
+            a.index = index;
+            a.arg = nargs ? args[0] : nullptr;
+            a.min_q = min_q;
+            a.seed = seed;
+            a.counter = counter;
+            a.occw = L.occ;
+            a.st = st;
+            if ((rc = launch_inject2<false>(ctx, a, L, cap))) return rc;
         }
+    }
+    // colour index of a phase: offsets (o_1 outer ... o_D inner); persistent
+    // wavefronts, one listed cell at a time each
+    for (int ph = 0; ph < NQ && !two; ++ph) {
         k_inject_wave<D><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
             G, cp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
             L.total + ph, st, PhaseInj{});
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;
+    if (ordered && stats && (rc = inject_wait_report(ctx))) return rc;
     return finish(ctx);
 }
 
@@ -713,6 +714,7 @@ int run_inject_phase(jp_ctx* ctx, const jp_cellgrid* gc, const jp_axis* xc,
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
     if ((rc = inject_lists(ctx, G.C, NQ, cap, 0, L))) return rc;
+    L.state = nullptr;  // colour launches only
     k_inject_scan<D, true><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, cptr3(coords, D),
                                                                         index, min_q, L);
     JP_LAUNCHED(ctx);

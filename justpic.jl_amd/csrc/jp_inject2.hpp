@@ -19,6 +19,27 @@
 // rounded distance in (neighbour cell, slot) order), and the field copy,
 // which reads the source lane's current value (an earlier injection's copy
 // when the nearest particle is one of them).
+//
+// The reference runs the cells in four colour launches (offsets (1,1),
+// (1,2), (2,1), (2,2), injection.jl:28-35) because a cell reads its 3x3
+// neighbourhood and writes itself.  ORDERED runs all four colour lists in
+// one launch: a listed cell first waits until every listed neighbour of an
+// earlier colour has published its result (per-cell state word: 0 not
+// listed, 1 listed, 2 done), which gives every read the value of the colour
+// order.  A later-colour listed neighbour waits for this cell in turn, so it
+// cannot write before this cell has read it.  Only the data that crosses
+// cells this way is made coherent across the chip, access by access: the
+// cell's stores of positions, field values and occupancy word are
+// device-scope (written through the XCD's L2), the state is stored after a
+// wait for them, and a cell reads a done neighbour's data with device-scope
+// loads (an agent-scope release / acquire fence would write back or drop
+// the whole L2 of the XCD for every cell).  The wavefronts
+// are persistent and all resident at once (the launch is capped at the
+// occupancy limit); wavefront w takes items w, w + W, ... of the colour-major
+// item order, so the lowest unfinished item always belongs to a running
+// wavefront whose dependencies are all lower items: every wait ends.  A wait
+// is still bounded (kInjectWaitMax polls) and reports a timeout instead of
+// spinning on.
 #pragma once
 
 #include <type_traits>
@@ -36,11 +57,44 @@ struct Inject2 {
     double* arg;       // the field moved with the particles, or null
     int min_q;         // particles wanted per quadrant
     uint64_t seed, counter;
-    const int32_t* list;   // this colour's listed cells
-    const int32_t* count;  // their number
+    const int32_t* list;   // listed cells (ORDERED: colour q's at list + q * cap)
+    const int32_t* count;  // their number (ORDERED: one per colour)
     uint64_t* occw;        // occupancy word of every cell (updated here)
     jp_stats* st;
+    int64_t cap;           // ORDERED: list stride between colours
+    uint32_t* state;       // ORDERED: per cell 0 not listed, 1 listed, 2 done
+    int* timeout;          // ORDERED: set when a dependency wait gave up
 };
+
+constexpr int kInjectWaitMax = 1 << 20;
+// s_waitcnt immediate of gfx9 for vmcnt(0) with expcnt and lgkmcnt left free
+constexpr int kInjVmcnt0 = 0x0F70;
+
+// device-scope (coherent across the XCDs' L2 caches) loads and stores
+__device__ inline uint32_t ld_dev(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_dev(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float ld_dev(const float* p) {
+    return __uint_as_float(ld_dev(reinterpret_cast<const uint32_t*>(p)));
+}
+__device__ inline double ld_dev(const double* p) {
+    return __longlong_as_double((long long)ld_dev(reinterpret_cast<const uint64_t*>(p)));
+}
+__device__ inline void st_dev(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev(float* p, float v) {
+    st_dev(reinterpret_cast<uint32_t*>(p), __float_as_uint(v));
+}
+__device__ inline void st_dev(double* p, double v) {
+    st_dev(reinterpret_cast<uint64_t*>(p), (uint64_t)__double_as_longlong(v));
+}
 
 // a uniform lane's double (v_readlane on both halves)
 __device__ inline double readlane_d(double v, int l) {
@@ -64,13 +118,23 @@ __device__ inline uint64_t lowest_bits(uint64_t m, int n) {
     return t;
 }
 
-template <bool SPLIT>
+template <bool SPLIT, bool ORDERED>
 __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
     using P = typename std::conditional<SPLIT, float, double>::type;
     constexpr int NB = 9, CENTER = 4;
     const int lane = threadIdx.x & 63;
     const int64_t nwv = (int64_t)gridDim.x * (kBlock / 64);
-    const int64_t total = *a.count;
+    int64_t tq[4] = {0, 0, 0, 0};
+    int64_t total = 0;
+    if constexpr (ORDERED) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            tq[q] = a.count[q];
+            total += tq[q];
+        }
+    } else {
+        total = *a.count;
+    }
     const Grid<2>& g = a.g;
     const int64_t C = g.C;
     const int K = g.K;
@@ -80,7 +144,18 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
     int64_t n_inj = 0, n_orph = 0;
     for (int64_t w = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); w < total;
          w += nwv) {
-        const int64_t c = __builtin_amdgcn_readfirstlane(a.list[w]);
+        int64_t at = w;
+        if constexpr (ORDERED) {
+            // item w in colour-major order: colour q, entry w - (counts before q)
+            int64_t r = w;
+            int q = 0;
+            while (q < 3 && r >= tq[q]) {
+                r -= tq[q];
+                ++q;
+            }
+            at = q * a.cap + r;
+        }
+        const int64_t c = __builtin_amdgcn_readfirstlane(a.list[at]);
         int64_t I[2];
         unlin<2>(g.ni, c, I);
         int64_t cn[NB];
@@ -91,9 +166,45 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
             okn[nb] = J0 >= 0 && J0 < g.ni[0] && J1 >= 0 && J1 < g.ni[1];
             cn[nb] = okn[nb] ? J0 + g.ni[0] * J1 : c;  // a valid cell either way
         }
+        bool coh[NB];  // a done neighbour's data: device-scope loads
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) coh[nb] = false;
+        if constexpr (ORDERED) {
+            // lane nb watches neighbour nb when it has an earlier colour
+            // (colour = 2 * (i mod 2) + (j mod 2), the launch order)
+            const int mine = (int)(((I[0] & 1) << 1) | (I[1] & 1));
+            int64_t wc = c;
+            bool need = false;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int64_t J0 = I[0] + nb % 3 - 1, J1 = I[1] + nb / 3 - 1;
+                const int col = (int)(((J0 & 1) << 1) | (J1 & 1));
+                if (lane == nb) {
+                    wc = cn[nb];
+                    need = okn[nb] && col < mine;
+                }
+            }
+            uint32_t sv = need ? ld_dev(a.state + wc) : 0u;
+            const uint64_t listed = __ballot(sv != 0u);
+            if (listed) {  // an earlier-colour neighbour injects in this call
+                int polls = 0;
+                while (__ballot(sv == 1u)) {
+                    if (++polls > kInjectWaitMax) {
+                        if (lane == 0) atomicOr(a.timeout, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    if (sv == 1u) sv = ld_dev(a.state + wc);
+                }
+            }
+            // (the loads below issue after the state values they depend on)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) coh[nb] = (listed >> nb) & 1;
+        }
         uint64_t o9[NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) o9[nb] = a.occw[cn[nb]];
+        for (int nb = 0; nb < NB; ++nb) o9[nb] = coh[nb] ? ld_dev(a.occw + cn[nb]) : a.occw[cn[nb]];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) o9[nb] = okn[nb] ? o9[nb] : 0ull;
         // positions of the neighbourhood's particles (active slots only: in
@@ -107,11 +218,21 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
             const bool take = (o9[nb] >> lane) & 1;
             const int64_t e2 = cn[nb] + C * lane;
             if constexpr (SPLIT) {
-                q[nb][0] = take ? a.u[0][e2] : 0.0f;
-                q[nb][1] = take ? a.u[1][e2] : 0.0f;
+                if (coh[nb]) {
+                    q[nb][0] = take ? ld_dev(a.u[0] + e2) : 0.0f;
+                    q[nb][1] = take ? ld_dev(a.u[1] + e2) : 0.0f;
+                } else {
+                    q[nb][0] = take ? a.u[0][e2] : 0.0f;
+                    q[nb][1] = take ? a.u[1][e2] : 0.0f;
+                }
             } else {
-                q[nb][0] = take ? a.x[0][e2] : 0.0;
-                q[nb][1] = take ? a.x[1][e2] : 0.0;
+                if (coh[nb]) {
+                    q[nb][0] = take ? ld_dev(a.x[0] + e2) : 0.0;
+                    q[nb][1] = take ? ld_dev(a.x[1] + e2) : 0.0;
+                } else {
+                    q[nb][0] = take ? a.x[0][e2] : 0.0;
+                    q[nb][1] = take ? a.x[1][e2] : 0.0;
+                }
             }
         }
         double fc = (has_arg && slot) ? a.arg[c + C * lane] : 0.0;
@@ -222,9 +343,13 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
             if (lane < ninj && src != 0x7fffffff && (src >> 6) != CENTER) {
                 const int nbw = src >> 6;
                 int64_t cs = cn[0];
+                bool cc = coh[0];
 #pragma unroll
-                for (int nb = 1; nb < NB; ++nb) cs = nb == nbw ? cn[nb] : cs;
-                nv = a.arg[cs + C * (src & 63)];
+                for (int nb = 1; nb < NB; ++nb) {
+                    cs = nb == nbw ? cn[nb] : cs;
+                    cc = nb == nbw ? coh[nb] : cc;
+                }
+                nv = cc ? ld_dev(a.arg + cs + C * (src & 63)) : a.arg[cs + C * (src & 63)];
             }
             // the copies in injection order: a source in this cell is read
             // from the current values (an earlier injection's copy included)
@@ -241,23 +366,84 @@ __global__ void __launch_bounds__(kBlock) k_inject2(Inject2 a) {
         n_inj += __popcll(injm);
         if ((injm >> lane) & 1) {
             const int64_t e = c + C * lane;
-            if constexpr (SPLIT) {
-                a.u[0][e] = q[CENTER][0];
-                a.u[1][e] = q[CENTER][1];
+            if constexpr (ORDERED) {
+                if constexpr (SPLIT) {
+                    st_dev(a.u[0] + e, q[CENTER][0]);
+                    st_dev(a.u[1] + e, q[CENTER][1]);
+                } else {
+                    st_dev(a.x[0] + e, q[CENTER][0]);
+                    st_dev(a.x[1] + e, q[CENTER][1]);
+                }
+                if (has_arg) st_dev(a.arg + e, fc);
             } else {
-                a.x[0][e] = q[CENTER][0];
-                a.x[1][e] = q[CENTER][1];
+                if constexpr (SPLIT) {
+                    a.u[0][e] = q[CENTER][0];
+                    a.u[1][e] = q[CENTER][1];
+                } else {
+                    a.x[0][e] = q[CENTER][0];
+                    a.x[1][e] = q[CENTER][1];
+                }
+                if (has_arg) a.arg[e] = fc;
             }
-            a.index[e] = 1;
-            if (has_arg) a.arg[e] = fc;
+            a.index[e] = 1;  // (read by later launches only)
         }
-        // later colour phases see this cell's final occupancy
-        if (lane == 0) a.occw[c] = o9[CENTER] | injm;
+        // later colours see this cell's final occupancy
+        if constexpr (ORDERED) {
+            if (lane == 0) st_dev(a.occw + c, o9[CENTER] | injm);
+            // publish: this cell's stores, and its loads of the neighbourhood,
+            // complete before its state says done
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(kInjVmcnt0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (lane == 0) st_dev(a.state + c, 2u);
+        } else {
+            if (lane == 0) a.occw[c] = o9[CENTER] | injm;
+        }
     }  // work list
     if (lane == 0) {
         stat_add(a.st, ST_INJECTED, n_inj);
         stat_add(a.st, ST_ORPHAN, n_orph);
     }
+}
+
+// The 2D injection's launches after the scan and the join: the four colours
+// in one ordered launch (default), or one launch per colour
+// (JP_OPT_INJECT_PHASES).  `a` holds everything but the lists.
+inline bool inject2_ordered(const jp_ctx* ctx) { return ctx->opt_inject_phases == 0; }
+
+template <bool SPLIT>
+int launch_inject2(jp_ctx* ctx, Inject2 a, const InjectLists& L, int64_t cap) {
+    if (inject2_ordered(ctx)) {
+        // every wavefront of the launch resident at once (each may wait on
+        // lower items of the others)
+        static int per_cu = -1;
+        if (per_cu < 0) {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_inject2<SPLIT, true>, kBlock,
+                                                             0) != hipSuccess ||
+                n < 1)
+                n = 1;
+            per_cu = n;
+        }
+        const int64_t most = (int64_t)per_cu * ctx->num_cu;
+        const int64_t want = inject_blocks(ctx, 4 * cap);
+        a.list = L.list;
+        a.count = L.total;
+        a.cap = cap;
+        a.state = L.state;
+        a.timeout = dinject(ctx);
+        k_inject2<SPLIT, true>
+            <<<(unsigned)(want < most ? want : most), kBlock, 0, ctx->stream>>>(a);
+        JP_LAUNCHED(ctx);
+        return JP_OK;
+    }
+    for (int ph = 0; ph < 4; ++ph) {
+        a.list = L.list + (int64_t)ph * cap;
+        a.count = L.total + ph;
+        k_inject2<SPLIT, false><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(a);
+        JP_LAUNCHED(ctx);
+    }
+    return JP_OK;
 }
 
 }  // namespace jp

@@ -578,6 +578,7 @@ __global__ void __launch_bounds__(kBlock) k_inj_scan_split(Grid<D> g, FPtr3 off,
             }
         }
     }
+    if (valid && L.state) L.state[c] = num < min_q ? 1u : 0u;
     int col = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) col = col * 2 + (int)(I[d] & 1);
@@ -1018,6 +1019,9 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     for (int d = 0; d < D; ++d) cap *= (G.ni[d] + 1) / 2;
     InjectLists L;
     if ((rc = inject_lists(ctx, G.C, NQ, cap, 0, L))) return rc;
+    const bool two = D == 2 && nargs <= 1;  // the 2D kernel (k_inject2)
+    const bool ordered = two && inject2_ordered(ctx);
+    if (!ordered) L.state = nullptr;
     FPtr3 fp = fptr3<D>(off);
     if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
     k_inj_scan_split<D><<<blocks_for(G.C), kBlock, 0, ctx->stream>>>(G, fp, index, min_q, L,
@@ -1027,35 +1031,32 @@ int run_inject(jp_ctx* ctx, const jp_cellgrid* gc, float* const* off, uint8_t* i
     ArgPtrs ap;
     for (int i = 0; i < JP_MAX_ARGS; ++i) ap.p[i] = i < nargs ? args[i] : nullptr;
     jp_stats* st = stats ? dstats(ctx) : nullptr;
-    for (int ph = 0; ph < NQ; ++ph) {
-        if constexpr (D == 2) {
-            if (nargs <= 1) {
-                Inject2 a{};
-                a.g = G;
-                a.u[0] = off[0];
-                a.u[1] = off[1];
-                a.h[0] = sg.h[0][0];
-                a.h[1] = sg.h[1][0];
-                a.index = index;
-                a.arg = nargs ? args[0] : nullptr;
-                a.min_q = min_q;
-                a.seed = seed;
-                a.counter = counter;
-                a.list = L.list + (int64_t)ph * cap;
-                a.count = L.total + ph;
-                a.occw = L.occ;
-                a.st = st;
-                k_inject2<true><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(a);
-                JP_LAUNCHED(ctx);
-                continue;
-            }
+    if constexpr (D == 2) {
+        if (two) {
+            Inject2 a{};
+            a.g = G;
+            a.u[0] = off[0];
+            a.u[1] = off[1];
+            a.h[0] = sg.h[0][0];
+            a.h[1] = sg.h[1][0];
+            a.index = index;
+            a.arg = nargs ? args[0] : nullptr;
+            a.min_q = min_q;
+            a.seed = seed;
+            a.counter = counter;
+            a.occw = L.occ;
+            a.st = st;
+            if ((rc = launch_inject2<true>(ctx, a, L, cap))) return rc;
         }
+    }
+    for (int ph = 0; ph < NQ && !two; ++ph) {
         k_inj_wave_split<D><<<(unsigned)inject_blocks(ctx, cap), kBlock, 0, ctx->stream>>>(
             G, sg, fp, index, nargs, ap, min_q, seed, counter, L.list + (int64_t)ph * cap,
             L.total + ph, st);
         JP_LAUNCHED(ctx);
     }
     if ((rc = stats_end(ctx, stats))) return rc;
+    if (ordered && stats && (rc = inject_wait_report(ctx))) return rc;
     return split_finish(ctx);
 }
 

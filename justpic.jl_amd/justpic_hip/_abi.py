@@ -19,7 +19,7 @@ JP_MAX_ARGS = 16
 JP_ENS_VARIANTS = 8
 JP_ADJ_MAX_EVERY = 16
 JP_OPT_SLOT_SPAN, JP_OPT_INJECT_WAVES, JP_OPT_GEOMETRY_RECHECK = 1, 2, 3
-JP_OPT_MOVE_PHASES, JP_OPT_ADVECT_SLOTS = 4, 5
+JP_OPT_MOVE_PHASES, JP_OPT_ADVECT_SLOTS, JP_OPT_INJECT_PHASES = 4, 5, 6
 JP_NOALLOC_MAX_CELLS = 1 << 21
 JP_MAX_PHASES = 16
 JP_INTERP_LINEAR, JP_INTERP_MQS, JP_INTERP_LINP = 0, 1, 2

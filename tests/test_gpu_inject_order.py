@@ -1,0 +1,128 @@
+"""The 2D injection's four colours in one launch, against the oracle and
+against one launch per colour (runs on an MI355X).
+
+In the single launch a listed cell waits for its listed neighbours of
+earlier colours, so the states here are built to have many of them: random
+occupancy with rectangles of empty or nearly empty cells, where every colour
+is listed next to every other (chains of four dependent cells).  Compared
+bit for bit with the oracle (FP64) and with the per-colour launches (FP64
+and split: masks, positions and the copied field), with the default number
+of persistent wavefronts, with 64 of them (each takes many cells, so a
+wavefront waits on cells that other wavefronts take later in their own
+order) and with one workgroup of four.  Two calls in a row on one context
+check that the per-cell state of one call does not leak into the next.
+"""
+import contextlib
+
+import numpy as np
+import pytest
+
+import justpic_hip as jp
+from justpic_hip import _abi
+from justpic_hip.grids import Axis, staggered_grid
+from oracle import oracle as O
+
+from helpers import clear, place
+from test_gpu_parity import assert_state_equal, to_dev
+from test_gpu_split import split_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-inject-order]", *a, flush=True)
+
+
+@contextlib.contextmanager
+def options(ctx, **kv):
+    ids = {"phases": _abi.JP_OPT_INJECT_PHASES, "waves": _abi.JP_OPT_INJECT_WAVES}
+    for k, v in kv.items():
+        ctx.set_option(ids[k], v)
+    try:
+        yield
+    finally:
+        for k in kv:
+            ctx.set_option(ids[k], 0)
+
+
+def clustered_state(n0, n1, K, seed, holes=40):
+    """Random occupancy (0..K particles per cell) with `holes` rectangles of
+    2..9 cells a side holding at most one particle per cell."""
+    xv = (Axis(np.arange(n0 + 1) / n0, "range"), Axis(np.arange(n1 + 1) / n1, "range"))
+    xi = staggered_grid(xv)
+    hp = O.HostParticles(xv, K, xi)
+    clear(hp)
+    rng = np.random.default_rng(seed)
+    cnt = rng.integers(0, K + 1, (n0, n1))
+    for _ in range(holes):
+        w, h = rng.integers(2, 10, 2)
+        i0, j0 = rng.integers(0, n0 - 1), rng.integers(0, n1 - 1)
+        cnt[i0:i0 + w, j0:j0 + h] = rng.integers(0, 2, cnt[i0:i0 + w, j0:j0 + h].shape)
+    for i in range(n0):
+        for j in range(n1):
+            for s in range(int(cnt[i, j])):
+                place(hp, (i, j), s, ((i + rng.uniform(0.02, 0.98)) / n0,
+                                      (j + rng.uniform(0.02, 0.98)) / n1))
+    return hp
+
+
+def run_pair(ctx, hp, T, minx, seed, **opt):
+    """The FP64 and split forms on copies of one state, with options."""
+    p = to_dev(ctx, hp, min_xcell=minx, seed=seed)
+    s = jp.to_split(p)
+    Td = jp.CellArray(ctx, hp.ni, hp.K)
+    Td.copy_from(T)
+    Ts = Td.copy()
+    with options(ctx, **opt):
+        st_d = jp.inject_particles_(p, (Td,), stats=True)
+        st_s = jp.inject_particles_(s, (Ts,), stats=True)
+    return p, s, Td, Ts, st_d, st_s
+
+
+@pytest.mark.parametrize("waves", [0, 64, 1])
+@pytest.mark.parametrize("shape", [(160, 96), (37, 211)])
+def test_inject_one_launch(ctx, shape, waves):
+    n0, n1 = shape
+    say("one launch", shape, "waves", waves)
+    K, minx, seed = 16, 8, 7
+    hp = clustered_state(n0, n1, K, seed=n0 + n1)
+    rng = np.random.default_rng(1)
+    T = np.where(hp.index == 1, rng.random(hp.S), -1.0 - rng.random(hp.S))
+    p, s, Td, Ts, st_d, st_s = run_pair(ctx, hp, T, minx, seed, waves=waves)
+    q, r, Qd, Qs, sq_d, sq_s = run_pair(ctx, hp, T, minx, seed, phases=1)
+    To = T.copy()
+    st_o = O.inject_particles(hp, [To], minx, seed=seed, counter=0)
+    say("stats", st_d, st_s, st_o)
+    assert st_o["injected"] > 1000
+    assert st_d == st_o and st_s == st_o and sq_d == st_o and sq_s == st_o
+    assert_state_equal(p, hp, exact=True)
+    assert_state_equal(q, hp, exact=True)
+    assert np.array_equal(Td.data.flat(), To, equal_nan=True)
+    assert np.array_equal(Qd.data.flat(), To, equal_nan=True)
+    # split: the same masks, offsets and copies as its per-colour launches
+    c1, i1 = split_state(s)
+    c2, i2 = split_state(r)
+    assert np.array_equal(i1, hp.index) and np.array_equal(i2, hp.index)
+    for d in range(2):
+        assert np.array_equal(c1[d], c2[d], equal_nan=True)
+    assert np.array_equal(Ts.data.flat(), Qs.data.flat(), equal_nan=True)
+
+
+def test_inject_calls_in_a_row(ctx):
+    """Two states with different listed cells, one after the other on one
+    context: each bit for bit against the oracle."""
+    say("calls in a row")
+    K, minx = 12, 8
+    for it, seed in enumerate((21, 22)):
+        hp = clustered_state(96, 64, K, seed=seed, holes=25)
+        rng = np.random.default_rng(seed)
+        T = np.where(hp.index == 1, rng.random(hp.S), np.nan)
+        p = to_dev(ctx, hp, min_xcell=minx, seed=seed)
+        Td = jp.CellArray(ctx, hp.ni, hp.K)
+        Td.copy_from(T)
+        st_d = jp.inject_particles_(p, (Td,), stats=True)
+        st_o = O.inject_particles(hp, [T], minx, seed=seed, counter=0)
+        say("call", it, st_d, st_o)
+        assert st_d == st_o
+        assert_state_equal(p, hp, exact=True)
+        assert np.array_equal(Td.data.flat(), T, equal_nan=True)
