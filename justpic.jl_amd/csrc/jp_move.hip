@@ -56,8 +56,9 @@ struct MoveScratch {
     uint64_t* arr;   // slots holding a particle that arrived during this call
     uint64_t* tch;   // slots that received a particle at some point
     uint8_t* code;   // per slot, 8 per word (word j*C + c): neighbour code / kDel, movers only
-    uint32_t* org;   // per slot: original slot index of an arrival (layout: org_at)
-    char* stg;       // staged movers by original slot o = c + C k (layout: stg_coord)
+    uint32_t* org;   // per slot: staged index of an arrival's record (layout: org_at)
+    char* stg;       // staged movers (layout: stg_coord; index: stage_index)
+    uint64_t* stm;   // 3D: each cell's staged movers (bit k), written by classify
     int64_t S;       // slots
     int64_t stride;  // 3D: record bytes (multiple of 8)
     int64_t foff;    // 3D: byte offset of the fields in a record; 2D: of the field planes
@@ -102,6 +103,19 @@ __device__ inline T& stg_coord(const MoveScratch& ms, int64_t o, int d) {
     else
         return reinterpret_cast<T*>(ms.stg)[o * D + d];
 }
+// Index of the staged record of a cell's mover: 2D by slot (o = c + C k, the
+// interleaved coordinates and field planes written coalesced across cells);
+// 3D packed per cell, the r-th staged mover of cell c (slot order) at
+// c K + r, so that a destination's arrivals from one origin cell share cache
+// lines (the 3D records are 24+ bytes, one line each when indexed by slot).
+// `below` is the cell's staged movers under slot k.
+template <int D>
+__device__ inline int64_t stage_index(int64_t c, int64_t C, int K, int k, uint64_t below) {
+    if constexpr (D == 3)
+        return c * K + __popcll(below);
+    else
+        return c + C * k;
+}
 template <int D>
 __device__ inline double& stg_field(const MoveScratch& ms, int64_t o, int a) {
     if constexpr (D == 3)
@@ -114,11 +128,10 @@ __device__ inline double& stg_field(const MoveScratch& ms, int64_t o, int a) {
 // Fields a0.. of the movers of one chunk of 8 slots, copied to the staging
 // area in one batch of independent loads (stg: chunk-local mover bits).
 template <int D>
-__device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint32_t stg,
-                                  int nargs, const ArgPtrs& args, const MoveScratch& ms,
-                                  int a0 = 0) {
+__device__ inline void stage_args(int64_t c, int64_t C, int K, int k0, uint32_t stg,
+                                  uint64_t staged, int nargs, const ArgPtrs& args,
+                                  const MoveScratch& ms, int a0 = 0) {
     if (!stg) return;
-    (void)S;
     for (int a = a0; a < nargs; ++a) {
         const double* __restrict__ src = args.p[a];
         double v[8];
@@ -126,7 +139,10 @@ __device__ inline void stage_args(int64_t c, int64_t C, int64_t S, int k0, uint3
         for (int j = 0; j < 8; ++j) v[j] = ((stg >> j) & 1) ? src[c + C * (k0 + j)] : 0.0;
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            if ((stg >> j) & 1) stg_field<D>(ms, c + C * (k0 + j), a) = v[j];
+            if ((stg >> j) & 1)
+                stg_field<D>(ms, stage_index<D>(c, C, K, k0 + j,
+                                                staged | ((uint64_t)(stg & ((1u << j) - 1)) << k0)),
+                             a) = v[j];
     }
 }
 
@@ -170,7 +186,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
     if (c >= g.C) return;
     const int kb = blockIdx.y * span;  // slot group kb..ke-1
     const int ke = kb + span < g.K ? kb + span : g.K;
-    const int64_t C = g.C, S = C * g.K;
+    const int64_t C = g.C;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
     // vertices x[I-1..I+2] and spacings dx[I-1..I+1] per dimension, loaded once:
@@ -194,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
     }
     const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
                                                 : active_slots_in(index, c, C, kb, ke);
-    uint64_t mov = 0;
+    uint64_t mov = 0, staged = 0;
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
     // loads and stores across neighbouring cells)
@@ -257,17 +273,21 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     cd = (uint8_t)n;
                     if (n != kCenter<D> && !inJ) cd |= kGap;
                 }
+                const int64_t so =
+                    stage_index<D>(c, C, g.K, k, staged | ((uint64_t)stg << k0));
 #pragma unroll
-                for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, c + C * k, d) = p[d];
+                for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, so, d) = p[d];
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args<D>(c, C, S, k0, stg, nargs, args, ms);
+        stage_args<D>(c, C, g.K, k0, stg, staged, nargs, args, ms);
+        staged |= (uint64_t)stg << k0;
     }
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
+    if constexpr (D == 3) ms.stm[c] = staged;  // (3D classify walks whole cells)
 }
 
 // ------------------------------------------------------------ 2. colour phases
@@ -323,6 +343,7 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
     uint64_t pend = ms.mov[S];
     uint64_t arrS = ms.arr[S];
     const uint64_t tchS = ms.tch[S];
+    const uint64_t stmS = D == 3 ? ms.stm[S] : 0ull;  // staged movers (their record index)
     uint64_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // movers' code bytes, 8 per word
     {
         const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S;
@@ -356,7 +377,9 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
         pend &= pend - 1;
         const uint64_t bit = 1ull << ip;
         const uint8_t code = reinterpret_cast<const uint8_t*>(&s_cw[ip >> 3][tid])[ip & 7];
-        uint32_t origin = S32 + C32 * (uint32_t)ip;
+        // a mover of this cell: its staged record; an arrival: the record
+        // its origin staged
+        uint32_t origin = (uint32_t)stage_index<D>(S, C, K, ip, stmS & (bit - 1));
         if (arrS & bit) {  // rare: an arrival of this call moves on
             origin = ms.org[org_at<D>(S32, (uint32_t)ip, C32, (uint32_t)K)];
             // wait here, on the rare path: otherwise the loop head waits for
@@ -948,12 +971,12 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
     if (c >= g.C) return;
     const int kb = blockIdx.y * span;  // slot group kb..ke-1
     const int ke = kb + span < g.K ? kb + span : g.K;
-    const int64_t C = g.C, S = C * g.K;
+    const int64_t C = g.C;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
     const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
                                                 : active_slots_in(index, c, C, kb, ke);
-    uint64_t mov = 0;
+    uint64_t mov = 0, staged = 0;
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     constexpr int B = 8;
     const double* __restrict__ f0 = nargs ? args.p[0] : nullptr;
@@ -1001,14 +1024,16 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 atomicOr(flag, kFlagFar);
                 atomicMax(flag + 1, jump);
                 cd = kFar;
+                const int64_t so = stage_index<D>(c, C, g.K, k, staged | ((uint64_t)stg << k0));
 #pragma unroll
-                for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, c + C * k, d) = ub[j][d];
+                for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, so, d) = ub[j][d];
                 stg |= 1u << j;
             } else {
                 cd = (uint8_t)n;
+                const int64_t so = stage_index<D>(c, C, g.K, k, staged | ((uint64_t)stg << k0));
 #pragma unroll
                 for (int d = 0; d < D; ++d)
-                    stg_coord<D, float>(ms, c + C * k, d) = (float)((double)ub[j][d] - (double)s[d]);
+                    stg_coord<D, float>(ms, so, d) = (float)((double)ub[j][d] - (double)s[d]);
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
@@ -1018,11 +1043,17 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
         if (stg && f0) {
 #pragma unroll
             for (int j = 0; j < B; ++j)
-                if ((stg >> j) & 1) stg_field<D>(ms, c + C * (k0 + j), 0) = fb[j];
+                if ((stg >> j) & 1)
+                    stg_field<D>(ms,
+                                 stage_index<D>(c, C, g.K, k0 + j,
+                                                staged | ((uint64_t)(stg & ((1u << j) - 1)) << k0)),
+                                 0) = fb[j];
         }
-        stage_args<D>(c, C, S, k0, stg, nargs, args, ms, 1);
+        stage_args<D>(c, C, g.K, k0, stg, staged, nargs, args, ms, 1);
+        staged |= (uint64_t)stg << k0;
     }
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
+    if constexpr (D == 3) ms.stm[c] = staged;  // (3D classify walks whole cells)
 }
 
 ArgPtrs argptrs(double* const* a, int n) {
@@ -1060,6 +1091,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     const size_t o_tch = off;  off = align_up(off + C * 8, 256);
     const size_t o_code = off; off = align_up(off + (size_t)C * 64, 256);
     const size_t o_org = off;  off = align_up(off + (size_t)S * 4, 256);
+    const size_t o_stm = off;  off = align_up(off + (D == 3 ? C * 8 : 0), 256);
     ms.S = S;
     if (D == 3) {
         ms.foff = (int64_t)align_up(sizeof(T) * D, 8);
@@ -1081,6 +1113,7 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     ms.tch = (uint64_t*)(base + o_tch);
     ms.code = (uint8_t*)(base + o_code);
     ms.org = (uint32_t*)(base + o_org);
+    ms.stm = (uint64_t*)(base + o_stm);
     ms.stg = base + o_stg;
 
     int* flag = dflags(ctx);
@@ -1106,15 +1139,18 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     // thread, no chunk's loads wait for an earlier chunk's stores)
     const int span = slot_span(ctx, blocks_for(C), g.K, 6144);
     const dim3 cgrid((unsigned)blocks_for(C), (unsigned)((g.K + span - 1) / span));
+    // 3D classify walks whole cells (it packs each cell's staged movers)
+    const int cspan = D == 3 ? g.K : span;
+    const dim3 kgrid((unsigned)blocks_for(C), (unsigned)((g.K + cspan - 1) / cspan));
     if constexpr (std::is_same<T, double>::value) {
         CPtr3 cc{{nullptr, nullptr, nullptr}};
         for (int d = 0; d < D; ++d) cc.p[d] = coords[d];
-        k_mv_classify<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms, flag,
-                                                           span);
+        k_mv_classify<D><<<kgrid, kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms, flag,
+                                                           cspan);
     } else {
         if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
-        k_mv_classify_split<D><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
-                                                                 flag, span, geo_ptr(ctx));
+        k_mv_classify_split<D><<<kgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
+                                                                 flag, cspan, geo_ptr(ctx));
     }
     if ((rc = launched())) return rc;
     int64_t total = 1;
