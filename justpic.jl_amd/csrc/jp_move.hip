@@ -96,41 +96,34 @@ __device__ inline int phase_of(const int64_t* I) {
 // planes (classify's stores stay contiguous across a wavefront's cells).  3D:
 // one record per slot, the D coordinates then the fields (3D move -7% split,
 // -20% FP64 against planes; full records were 3% slower in 2D).
+// Staged movers: one record per mover (coordinates, then the fields at
+// byte offset foff), the r-th staged mover of cell c (slot order) at record
+// c K + r.  Packed per cell, a destination's arrivals from one origin cell
+// share cache lines in the scatter's gathers, and classify writes only the
+// movers' bytes (indexed by slot, every record was a cache line of its own
+// and the staging writes touched every line of a slot-sized array).
+// `below` is the cell's staged movers under the mover's slot.
+template <int D>
+__device__ inline int64_t stage_index(int64_t c, int K, uint64_t below) {
+    return c * K + __popcll(below);
+}
 template <int D, typename T>
 __device__ inline T& stg_coord(const MoveScratch& ms, int64_t o, int d) {
-    if constexpr (D == 3)
-        return reinterpret_cast<T*>(ms.stg + o * ms.stride)[d];
-    else
-        return reinterpret_cast<T*>(ms.stg)[o * D + d];
-}
-// Index of the staged record of a cell's mover: 2D by slot (o = c + C k, the
-// interleaved coordinates and field planes written coalesced across cells);
-// 3D packed per cell, the r-th staged mover of cell c (slot order) at
-// c K + r, so that a destination's arrivals from one origin cell share cache
-// lines (the 3D records are 24+ bytes, one line each when indexed by slot).
-// `below` is the cell's staged movers under slot k.
-template <int D>
-__device__ inline int64_t stage_index(int64_t c, int64_t C, int K, int k, uint64_t below) {
-    if constexpr (D == 3)
-        return c * K + __popcll(below);
-    else
-        return c + C * k;
+    return reinterpret_cast<T*>(ms.stg + o * ms.stride)[d];
 }
 template <int D>
 __device__ inline double& stg_field(const MoveScratch& ms, int64_t o, int a) {
-    if constexpr (D == 3)
-        return reinterpret_cast<double*>(ms.stg + o * ms.stride + ms.foff)[a];
-    else
-        return reinterpret_cast<double*>(ms.stg + ms.foff)[a * ms.S + o];
+    return reinterpret_cast<double*>(ms.stg + o * ms.stride + ms.foff)[a];
 }
 
 // ------------------------------------------------------------ 1. classify
 // Fields a0.. of the movers of one chunk of 8 slots, copied to the staging
 // area in one batch of independent loads (stg: chunk-local mover bits).
+// (o0: the record of the chunk's first staged mover)
 template <int D>
-__device__ inline void stage_args(int64_t c, int64_t C, int K, int k0, uint32_t stg,
-                                  uint64_t staged, int nargs, const ArgPtrs& args,
-                                  const MoveScratch& ms, int a0 = 0) {
+__device__ inline void stage_args(int64_t c, int64_t C, int k0, uint32_t stg, int64_t o0,
+                                  int nargs, const ArgPtrs& args, const MoveScratch& ms,
+                                  int a0 = 0) {
     if (!stg) return;
     for (int a = a0; a < nargs; ++a) {
         const double* __restrict__ src = args.p[a];
@@ -139,10 +132,58 @@ __device__ inline void stage_args(int64_t c, int64_t C, int K, int k0, uint32_t 
         for (int j = 0; j < 8; ++j) v[j] = ((stg >> j) & 1) ? src[c + C * (k0 + j)] : 0.0;
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            if ((stg >> j) & 1)
-                stg_field<D>(ms, stage_index<D>(c, C, K, k0 + j,
-                                                staged | ((uint64_t)(stg & ((1u << j) - 1)) << k0)),
-                             a) = v[j];
+            if ((stg >> j) & 1) stg_field<D>(ms, o0 + __popc(stg & ((1u << j) - 1)), a) = v[j];
+    }
+}
+
+// Classify's mapping.  3D: one thread per cell, whole cells (chunks of 8
+// slot planes in lockstep across the wavefront).  2D: a workgroup per 64
+// cells and one wavefront per 8-slot chunk (enough wavefronts to fill the
+// chip at 512^2, loads coalesced along the slot planes); the chunk-threads
+// of a cell add up their staged counts in LDS for the packed record index.
+constexpr int kClassifyMax = 512;  // 2D: 64 x up to 8 chunks
+template <int D>
+struct ClassifySlots {
+    int64_t c;
+    int kb, ke;
+    bool live;
+};
+template <int D>
+__device__ inline ClassifySlots<D> classify_slots(const Grid<D>& g) {
+    ClassifySlots<D> r;
+    if constexpr (D == 2) {
+        r.c = blockIdx.x * (int64_t)64 + (threadIdx.x & 63);
+        r.kb = (int)(threadIdx.x >> 6) * 8;
+        r.ke = r.kb + 8 < g.K ? r.kb + 8 : g.K;
+        r.live = r.c < g.C;
+    } else {
+        r.c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+        r.kb = 0;
+        r.ke = g.K;
+        r.live = r.c < g.C;
+    }
+    return r;
+}
+// The record index of the chunk's first staged mover.  2D: the counts of
+// the cell's lower chunks, through LDS (every thread of the block calls
+// this once); 3D: the running count of the thread's own cell.
+template <int D>
+__device__ inline int64_t chunk_record(int64_t c, int K, int k0, uint32_t stg, uint64_t staged,
+                                       int (&s_cnt)[8][64]) {
+    if constexpr (D == 2) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        s_cnt[w][lane] = __popc(stg);
+        __syncthreads();
+        int base = 0;
+        for (int v = 0; v < w; ++v) base += s_cnt[v][lane];
+        (void)k0;
+        (void)staged;
+        return c * K + base;
+    } else {
+        (void)s_cnt;
+        (void)k0;
+        (void)stg;
+        return c * K + __popcll(staged);
     }
 }
 
@@ -178,14 +219,15 @@ __device__ inline void store_cell_words(const MoveScratch& ms, int64_t c, int kb
 // index j*C + c (coalesced stores here), so the colour phases load a cell's
 // codes in one round of word loads.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
-                                                        const uint8_t* __restrict__ index,
-                                                        int nargs, ArgPtrs args, MoveScratch ms,
-                                                        int* flag, int span) {
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= g.C) return;
-    const int kb = blockIdx.y * span;  // slot group kb..ke-1
-    const int ke = kb + span < g.K ? kb + span : g.K;
+__global__ void __launch_bounds__(kClassifyMax) k_mv_classify(Grid<D> g, CPtr3 coords,
+                                                              const uint8_t* __restrict__ index,
+                                                              int nargs, ArgPtrs args,
+                                                              MoveScratch ms, int* flag) {
+    __shared__ int s_cnt[8][64];
+    const ClassifySlots<D> cs = classify_slots<D>(g);
+    if (D == 3 && !cs.live) return;  // (2D: every thread reaches the block's barrier)
+    const int64_t c = cs.live ? cs.c : 0;
+    const int kb = cs.kb, ke = cs.ke;  // slot chunk(s) kb..ke-1
     const int64_t C = g.C;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
@@ -208,8 +250,9 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
         dlo[d] = A.x[0];
         dhi[d] = A.x[n - 1];
     }
-    const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
-                                                : active_slots_in(index, c, C, kb, ke);
+    const uint64_t occ = !cs.live ? 0ull
+                         : (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                  : active_slots_in(index, c, C, kb, ke);
     uint64_t mov = 0, staged = 0;
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     // chunks of slot planes walked in lockstep by the wavefront (coalesced
@@ -273,21 +316,32 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify(Grid<D> g, CPtr3 coords,
                     cd = (uint8_t)n;
                     if (n != kCenter<D> && !inJ) cd |= kGap;
                 }
-                const int64_t so =
-                    stage_index<D>(c, C, g.K, k, staged | ((uint64_t)stg << k0));
-#pragma unroll
-                for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, so, d) = p[d];
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        stage_args<D>(c, C, g.K, k0, stg, staged, nargs, args, ms);
+        // the chunk's staged movers, packed after the cell's earlier ones
+        const int64_t o0 = chunk_record<D>(c, g.K, k0, stg, staged, s_cnt);
+        if (cs.live) {
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if ((stg >> j) & 1) {
+                    const int64_t o = o0 + __popc(stg & ((1u << j) - 1));
+#pragma unroll
+                    for (int d = 0; d < D; ++d) stg_coord<D, double>(ms, o, d) = pb[j][d];
+                }
+            stage_args<D>(c, C, k0, stg, o0, nargs, args, ms);
+        }
         staged |= (uint64_t)stg << k0;
     }
+    if (!cs.live) return;
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
-    if constexpr (D == 3) ms.stm[c] = staged;  // (3D classify walks whole cells)
+    if constexpr (D == 3)
+        ms.stm[c] = staged;
+    else
+        reinterpret_cast<uint8_t*>(ms.stm + c)[kb >> 3] = (uint8_t)(staged >> kb);
 }
 
 // ------------------------------------------------------------ 2. colour phases
@@ -343,7 +397,7 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
     uint64_t pend = ms.mov[S];
     uint64_t arrS = ms.arr[S];
     const uint64_t tchS = ms.tch[S];
-    const uint64_t stmS = D == 3 ? ms.stm[S] : 0ull;  // staged movers (their record index)
+    const uint64_t stmS = ms.stm[S];  // staged movers (their record index)
     uint64_t cw[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // movers' code bytes, 8 per word
     {
         const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S;
@@ -379,7 +433,7 @@ __device__ inline void phase_cell(const Grid<D>& g, const MoveScratch& ms, const
         const uint8_t code = reinterpret_cast<const uint8_t*>(&s_cw[ip >> 3][tid])[ip & 7];
         // a mover of this cell: its staged record; an arrival: the record
         // its origin staged
-        uint32_t origin = (uint32_t)stage_index<D>(S, C, K, ip, stmS & (bit - 1));
+        uint32_t origin = (uint32_t)stage_index<D>(S, K, stmS & (bit - 1));
         if (arrS & bit) {  // rare: an arrival of this call moves on
             origin = ms.org[org_at<D>(S32, (uint32_t)ip, C32, (uint32_t)K)];
             // wait here, on the rare path: otherwise the loop head waits for
@@ -647,21 +701,22 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
         cx = vx <= bx ? (bx - vx) / 3 + 1 : 0;
         cy = vy <= by ? (by - vy) / 3 + 1 : 0;
     };
-    auto load_codes = [&](uint32_t S32, uint64_t (&w)[8]) {
+    auto load_codes = [&](uint32_t S32, uint64_t (&w)[8], uint64_t& st) {
         const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S32;
 #pragma unroll
         for (int j = 0; j < 8; ++j) w[j] = j < nw ? cp[(int64_t)j * C32] : 0ull;
+        st = ms.stm[S32];
     };
     auto cell_at = [&](int vx, int vy, int cx, int t) {
         return (uint32_t)(vx + 3 * (t % cx)) + (uint32_t)n0 * (uint32_t)(vy + 3 * (t / cx));
     };
     // the code words of each thread's first cell of a phase are loaded one
     // phase ahead, so that they are in flight during the phase before
-    uint64_t nxt[8];
+    uint64_t nxt[8], nstm = 0;  // (and the cell's staged movers: their record index)
     {
         int vx, vy, cx, cy;
         box(0, vx, vy, cx, cy);
-        if ((int)threadIdx.x < cx * cy) load_codes(cell_at(vx, vy, cx, threadIdx.x), nxt);
+        if ((int)threadIdx.x < cx * cy) load_codes(cell_at(vx, vy, cx, threadIdx.x), nxt, nstm);
     }
     for (int p = 0; p < kNB<2>; ++p) {
         const int col0 = p / 3, col1 = p % 3;
@@ -672,10 +727,12 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
             for (int j = 0; j < 8; ++j)
                 if (j < nw) s_code[j][threadIdx.x] = nxt[j];
         }
+        const uint64_t stmv = nstm;
         if (p + 1 < kNB<2>) {
             int wx, wy, dx_, dy_;
             box(p + 1, wx, wy, dx_, dy_);
-            if ((int)threadIdx.x < dx_ * dy_) load_codes(cell_at(wx, wy, dx_, threadIdx.x), nxt);
+            if ((int)threadIdx.x < dx_ * dy_)
+                load_codes(cell_at(wx, wy, dx_, threadIdx.x), nxt, nstm);
         }
         const int t = threadIdx.x;  // at most one cell of the colour per thread
         uint64_t pend = 0;
@@ -704,7 +761,8 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
                 uint32_t cd = sc[(ip >> 3) * (BS * 8) + (ip & 7)];
                 const uint64_t bit = 1ull << ip;
                 occC &= ~bit;  // empty_particle! on the source slot
-                uint32_t origin = S32 + C32 * (uint32_t)ip;
+                // a mover of this cell: its packed record
+                uint32_t origin = S32 * (uint32_t)K + (uint32_t)__popcll(stmv & (bit - 1));
                 if (arrS & bit) {  // rare: an arrival of this call is re-slotted
                     origin = ms.org[org_at<2>(S32, (uint32_t)ip, C32, (uint32_t)K)];
                     // waited for here, on the rare path: a wait at the loop
@@ -958,24 +1016,25 @@ __global__ void __launch_bounds__(kFarBlock) k_mv_far(Grid<D> g, MoveScratch ms,
 // is the cell + floor(u) (deleted when that lies outside the grid), and the
 // staged offset is renormalised to the destination, u - floor(u), in FP64.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D, float> off,
-                                                              const uint8_t* __restrict__ index,
-                                                              int nargs, ArgPtrs args,
-                                                              MoveScratch ms, int* flag, int span,
-                                                              const SplitGeom* __restrict__ geo) {
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kClassifyMax)
+    k_mv_classify_split(Grid<D> g, TPtr3<D, float> off, const uint8_t* __restrict__ index,
+                        int nargs, ArgPtrs args, MoveScratch ms, int* flag,
+                        const SplitGeom* __restrict__ geo) {
+    __shared__ int s_cnt[8][64];
+    const ClassifySlots<D> cs = classify_slots<D>(g);
     if (geo->bad) {  // grid rejected: the later kernels of this move return at once
-        if (c == 0 && blockIdx.y == 0) atomicOr(flag, kFlagRejected);
-        return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flag, kFlagRejected);
+        return;  // (uniform over the block)
     }
-    if (c >= g.C) return;
-    const int kb = blockIdx.y * span;  // slot group kb..ke-1
-    const int ke = kb + span < g.K ? kb + span : g.K;
+    if (D == 3 && !cs.live) return;  // (2D: every thread reaches the block's barrier)
+    const int64_t c = cs.live ? cs.c : 0;
+    const int kb = cs.kb, ke = cs.ke;  // slot chunk(s) kb..ke-1
     const int64_t C = g.C;
     int64_t I[D];
     unlin<D>(g.ni, c, I);
-    const uint64_t occ = (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
-                                                : active_slots_in(index, c, C, kb, ke);
+    const uint64_t occ = !cs.live ? 0ull
+                         : (kb == 0 && ke == g.K) ? active_slots(index, c, C, g.K)
+                                                  : active_slots_in(index, c, C, kb, ke);
     uint64_t mov = 0, staged = 0;
     uint64_t* __restrict__ cw = reinterpret_cast<uint64_t*>(ms.code) + c;
     constexpr int B = 8;
@@ -1024,36 +1083,39 @@ __global__ void __launch_bounds__(kBlock) k_mv_classify_split(Grid<D> g, TPtr3<D
                 atomicOr(flag, kFlagFar);
                 atomicMax(flag + 1, jump);
                 cd = kFar;
-                const int64_t so = stage_index<D>(c, C, g.K, k, staged | ((uint64_t)stg << k0));
-#pragma unroll
-                for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, so, d) = ub[j][d];
                 stg |= 1u << j;
             } else {
                 cd = (uint8_t)n;
-                const int64_t so = stage_index<D>(c, C, g.K, k, staged | ((uint64_t)stg << k0));
 #pragma unroll
-                for (int d = 0; d < D; ++d)
-                    stg_coord<D, float>(ms, so, d) = (float)((double)ub[j][d] - (double)s[d]);
+                for (int d = 0; d < D; ++d)  // the staged value: the offset in the new cell
+                    ub[j][d] = (float)((double)ub[j][d] - (double)s[d]);
                 stg |= 1u << j;
             }
             word |= (uint64_t)cd << (8 * j);
         }
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
-        if (stg && f0) {
+        // the chunk's staged movers, packed after the cell's earlier ones
+        const int64_t o0 = chunk_record<D>(c, g.K, k0, stg, staged, s_cnt);
+        if (cs.live) {
 #pragma unroll
             for (int j = 0; j < B; ++j)
-                if ((stg >> j) & 1)
-                    stg_field<D>(ms,
-                                 stage_index<D>(c, C, g.K, k0 + j,
-                                                staged | ((uint64_t)(stg & ((1u << j) - 1)) << k0)),
-                                 0) = fb[j];
+                if ((stg >> j) & 1) {
+                    const int64_t o = o0 + __popc(stg & ((1u << j) - 1));
+#pragma unroll
+                    for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, o, d) = ub[j][d];
+                    if (f0) stg_field<D>(ms, o, 0) = fb[j];
+                }
+            stage_args<D>(c, C, k0, stg, o0, nargs, args, ms, 1);
         }
-        stage_args<D>(c, C, g.K, k0, stg, staged, nargs, args, ms, 1);
         staged |= (uint64_t)stg << k0;
     }
+    if (!cs.live) return;
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
-    if constexpr (D == 3) ms.stm[c] = staged;  // (3D classify walks whole cells)
+    if constexpr (D == 3)
+        ms.stm[c] = staged;
+    else
+        reinterpret_cast<uint8_t*>(ms.stm + c)[kb >> 3] = (uint8_t)(staged >> kb);
 }
 
 ArgPtrs argptrs(double* const* a, int n) {
@@ -1091,17 +1153,11 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     const size_t o_tch = off;  off = align_up(off + C * 8, 256);
     const size_t o_code = off; off = align_up(off + (size_t)C * 64, 256);
     const size_t o_org = off;  off = align_up(off + (size_t)S * 4, 256);
-    const size_t o_stm = off;  off = align_up(off + (D == 3 ? C * 8 : 0), 256);
+    const size_t o_stm = off;  off = align_up(off + C * 8, 256);
     ms.S = S;
-    if (D == 3) {
-        ms.foff = (int64_t)align_up(sizeof(T) * D, 8);
-        ms.stride = ms.foff + 8 * nargs;
-    } else {
-        ms.foff = (int64_t)align_up((size_t)S * sizeof(T) * D, 256);
-        ms.stride = 0;
-    }
-    const size_t stg_bytes = D == 3 ? (size_t)S * (size_t)ms.stride
-                                    : (size_t)ms.foff + (size_t)S * 8 * nargs;
+    ms.foff = (int64_t)align_up(sizeof(T) * D, 8);
+    ms.stride = ms.foff + 8 * nargs;
+    const size_t stg_bytes = (size_t)S * (size_t)ms.stride;
     const size_t o_stg = off;  off = align_up(off + stg_bytes, 256);
     int rc = ensure_scratch(ctx, off);
     if (rc) return rc;
@@ -1139,18 +1195,19 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     // thread, no chunk's loads wait for an earlier chunk's stores)
     const int span = slot_span(ctx, blocks_for(C), g.K, 6144);
     const dim3 cgrid((unsigned)blocks_for(C), (unsigned)((g.K + span - 1) / span));
-    // 3D classify walks whole cells (it packs each cell's staged movers)
-    const int cspan = D == 3 ? g.K : span;
-    const dim3 kgrid((unsigned)blocks_for(C), (unsigned)((g.K + cspan - 1) / cspan));
+    // classify (classify_slots): 2D a workgroup of one wavefront per 8-slot
+    // chunk for each 64 cells, 3D a thread per cell
+    const unsigned kblocks = (unsigned)(D == 2 ? (C + 63) / 64 : blocks_for(C));
+    const unsigned kthreads = (unsigned)(D == 2 ? 64 * ((g.K + 7) / 8) : kBlock);
     if constexpr (std::is_same<T, double>::value) {
         CPtr3 cc{{nullptr, nullptr, nullptr}};
         for (int d = 0; d < D; ++d) cc.p[d] = coords[d];
-        k_mv_classify<D><<<kgrid, kBlock, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms, flag,
-                                                           cspan);
+        k_mv_classify<D><<<kblocks, kthreads, 0, ctx->stream>>>(g, cc, index, nargs, ap, ms,
+                                                               flag);
     } else {
         if ((rc = split_geometry(ctx, gc, nullptr))) return rc;
-        k_mv_classify_split<D><<<kgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms,
-                                                                 flag, cspan, geo_ptr(ctx));
+        k_mv_classify_split<D><<<kblocks, kthreads, 0, ctx->stream>>>(g, cp, index, nargs, ap,
+                                                                     ms, flag, geo_ptr(ctx));
     }
     if ((rc = launched())) return rc;
     int64_t total = 1;
