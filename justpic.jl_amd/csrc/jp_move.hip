@@ -115,6 +115,24 @@ template <int D>
 __device__ inline double& stg_field(const MoveScratch& ms, int64_t o, int a) {
     return reinterpret_cast<double*>(ms.stg + o * ms.stride + ms.foff)[a];
 }
+// The 2D split record with one field is 16 bytes (two float offsets, the
+// field): written and read with one 16-byte access each.
+__device__ inline void stg_put16(const MoveScratch& ms, int64_t o, float u0, float u1, double f) {
+    const uint64_t b = (uint64_t)__double_as_longlong(f);
+    uint4 v;
+    v.x = __float_as_uint(u0);
+    v.y = __float_as_uint(u1);
+    v.z = (uint32_t)b;
+    v.w = (uint32_t)(b >> 32);
+    *reinterpret_cast<uint4*>(ms.stg + o * 16) = v;
+}
+__device__ inline void stg_get16(const MoveScratch& ms, int64_t o, float& u0, float& u1,
+                                 double& f) {
+    const uint4 v = *reinterpret_cast<const uint4*>(ms.stg + o * 16);
+    u0 = __uint_as_float(v.x);
+    u1 = __uint_as_float(v.y);
+    f = __longlong_as_double((long long)(((uint64_t)v.w << 32) | v.z));
+}
 
 // ------------------------------------------------------------ 1. classify
 // Fields a0.. of the movers of one chunk of 8 slots, copied to the staging
@@ -844,7 +862,8 @@ struct TPtr3 {
     T* p[3];
 };
 
-template <int D, typename T>
+// REC16: 2D split with one field (16-byte records, one load per arrival)
+template <int D, typename T, bool REC16 = false>
 __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> coords,
                                                        uint8_t* __restrict__ index, int nargs,
                                                        ArgPtrs args, MoveScratch ms,
@@ -898,6 +917,26 @@ __global__ void __launch_bounds__(kBlock) k_mv_scatter(Grid<D> g, TPtr3<D, T> co
                 o[j] = ((ab >> j) & 1) ? ms.org[org_at<D>((uint32_t)c, (uint32_t)(k0 + j),
                                                           (uint32_t)C, (uint32_t)g.K)]
                                        : 0u;
+        }
+        if constexpr (REC16) {
+            float u0[8], u1[8];
+            double fv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                u0[j] = u1[j] = __builtin_nanf("");
+                fv[j] = dnan();
+                if ((ab >> j) & 1) stg_get16(ms, o[j], u0[j], u1[j], fv[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (!((wb >> j) & 1)) continue;
+                const int64_t e = c + C * (k0 + j);
+                coords.p[0][e] = (T)u0[j];
+                coords.p[D - 1][e] = (T)u1[j];
+                index[e] = (uint8_t)((ab >> j) & 1);
+                args.p[0][e] = fv[j];
+            }
+            continue;
         }
         T v[8][D];
 #pragma unroll
@@ -1098,15 +1137,23 @@ __global__ void __launch_bounds__(kClassifyMax)
         // the chunk's staged movers, packed after the cell's earlier ones
         const int64_t o0 = chunk_record<D>(c, g.K, k0, stg, staged, s_cnt);
         if (cs.live) {
+            if (D == 2 && nargs == 1) {  // 16-byte records (the scatter reads them whole)
 #pragma unroll
-            for (int j = 0; j < B; ++j)
-                if ((stg >> j) & 1) {
-                    const int64_t o = o0 + __popc(stg & ((1u << j) - 1));
+                for (int j = 0; j < B; ++j)
+                    if ((stg >> j) & 1)
+                        stg_put16(ms, o0 + __popc(stg & ((1u << j) - 1)), ub[j][0],
+                                  ub[j][D - 1], fb[j]);
+            } else {
 #pragma unroll
-                    for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, o, d) = ub[j][d];
-                    if (f0) stg_field<D>(ms, o, 0) = fb[j];
-                }
-            stage_args<D>(c, C, k0, stg, o0, nargs, args, ms, 1);
+                for (int j = 0; j < B; ++j)
+                    if ((stg >> j) & 1) {
+                        const int64_t o = o0 + __popc(stg & ((1u << j) - 1));
+#pragma unroll
+                        for (int d = 0; d < D; ++d) stg_coord<D, float>(ms, o, d) = ub[j][d];
+                        if (f0) stg_field<D>(ms, o, 0) = fb[j];
+                    }
+                stage_args<D>(c, C, k0, stg, o0, nargs, args, ms, 1);
+            }
         }
         staged |= (uint64_t)stg << k0;
     }
@@ -1243,8 +1290,16 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     k_mv_far<D, T><<<1, kFarBlock, 0, ctx->stream>>>(g, ms, flag, st);
     if ((rc = launched())) return rc;
     const SplitGeom* geo = std::is_same<T, double>::value ? nullptr : geo_ptr(ctx);
-    k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, geo,
-                                                         span);
+    bool rec16 = false;  // 2D split, one field: 16-byte records
+    if constexpr (D == 2 && std::is_same<T, float>::value) {
+        rec16 = nargs == 1;
+        if (rec16)
+            k_mv_scatter<D, T, true><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap,
+                                                                        ms, geo, span);
+    }
+    if (!rec16)
+        k_mv_scatter<D, T><<<cgrid, kBlock, 0, ctx->stream>>>(g, cp, index, nargs, ap, ms, geo,
+                                                             span);
     if ((rc = launched())) return rc;
 
     if ((rc = stats_end(ctx, stats))) return rc;
