@@ -154,11 +154,14 @@ __device__ inline void stage_args(int64_t c, int64_t C, int k0, uint32_t stg, in
     }
 }
 
-// Classify's mapping.  3D: one thread per cell, whole cells (chunks of 8
-// slot planes in lockstep across the wavefront).  2D: a workgroup per 64
-// cells and one wavefront per 8-slot chunk (enough wavefronts to fill the
-// chip at 512^2, loads coalesced along the slot planes); the chunk-threads
-// of a cell add up their staged counts in LDS for the packed record index.
+// Classify's mapping.  CHUNKS (2D, and the 3D split form): a workgroup per
+// 64 cells and one wavefront per 8-slot chunk (enough wavefronts to fill the
+// chip at 512^2, loads coalesced along the slot planes, no chunk's loads
+// behind an earlier chunk's stores); the chunk-threads of a cell add up
+// their staged counts in LDS for the packed record index.  Otherwise (the
+// 3D FP64 form, whose 200 registers per lane do not fit such workgroups
+// well): one thread per cell, whole cells (chunks of 8 slot planes in
+// lockstep across the wavefront).
 constexpr int kClassifyMax = 512;  // 2D: 64 x up to 8 chunks
 template <int D>
 struct ClassifySlots {
@@ -166,10 +169,10 @@ struct ClassifySlots {
     int kb, ke;
     bool live;
 };
-template <int D>
+template <int D, bool CHUNKS>
 __device__ inline ClassifySlots<D> classify_slots(const Grid<D>& g) {
     ClassifySlots<D> r;
-    if constexpr (D == 2) {
+    if constexpr (CHUNKS) {
         r.c = blockIdx.x * (int64_t)64 + (threadIdx.x & 63);
         r.kb = (int)(threadIdx.x >> 6) * 8;
         r.ke = r.kb + 8 < g.K ? r.kb + 8 : g.K;
@@ -182,13 +185,13 @@ __device__ inline ClassifySlots<D> classify_slots(const Grid<D>& g) {
     }
     return r;
 }
-// The record index of the chunk's first staged mover.  2D: the counts of
-// the cell's lower chunks, through LDS (every thread of the block calls
-// this once); 3D: the running count of the thread's own cell.
-template <int D>
+// The record index of the chunk's first staged mover.  CHUNKS: the counts
+// of the cell's lower chunks, through LDS (every thread of the block calls
+// this once); else the running count of the thread's own cell.
+template <bool CHUNKS>
 __device__ inline int64_t chunk_record(int64_t c, int K, int k0, uint32_t stg, uint64_t staged,
                                        int (&s_cnt)[8][64]) {
-    if constexpr (D == 2) {
+    if constexpr (CHUNKS) {
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         s_cnt[w][lane] = __popc(stg);
         __syncthreads();
@@ -241,9 +244,10 @@ __global__ void __launch_bounds__(kClassifyMax) k_mv_classify(Grid<D> g, CPtr3 c
                                                               const uint8_t* __restrict__ index,
                                                               int nargs, ArgPtrs args,
                                                               MoveScratch ms, int* flag) {
+    constexpr bool CH = D == 2;
     __shared__ int s_cnt[8][64];
-    const ClassifySlots<D> cs = classify_slots<D>(g);
-    if (D == 3 && !cs.live) return;  // (2D: every thread reaches the block's barrier)
+    const ClassifySlots<D> cs = classify_slots<D, CH>(g);
+    if (!CH && !cs.live) return;  // (CH: every thread reaches the block's barrier)
     const int64_t c = cs.live ? cs.c : 0;
     const int kb = cs.kb, ke = cs.ke;  // slot chunk(s) kb..ke-1
     const int64_t C = g.C;
@@ -341,7 +345,7 @@ __global__ void __launch_bounds__(kClassifyMax) k_mv_classify(Grid<D> g, CPtr3 c
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
         // the chunk's staged movers, packed after the cell's earlier ones
-        const int64_t o0 = chunk_record<D>(c, g.K, k0, stg, staged, s_cnt);
+        const int64_t o0 = chunk_record<CH>(c, g.K, k0, stg, staged, s_cnt);
         if (cs.live) {
 #pragma unroll
             for (int j = 0; j < B; ++j)
@@ -356,10 +360,10 @@ __global__ void __launch_bounds__(kClassifyMax) k_mv_classify(Grid<D> g, CPtr3 c
     }
     if (!cs.live) return;
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
-    if constexpr (D == 3)
-        ms.stm[c] = staged;
-    else
+    if constexpr (CH)
         reinterpret_cast<uint8_t*>(ms.stm + c)[kb >> 3] = (uint8_t)(staged >> kb);
+    else
+        ms.stm[c] = staged;
 }
 
 // ------------------------------------------------------------ 2. colour phases
@@ -1060,12 +1064,11 @@ __global__ void __launch_bounds__(kClassifyMax)
                         int nargs, ArgPtrs args, MoveScratch ms, int* flag,
                         const SplitGeom* __restrict__ geo) {
     __shared__ int s_cnt[8][64];
-    const ClassifySlots<D> cs = classify_slots<D>(g);
+    const ClassifySlots<D> cs = classify_slots<D, true>(g);
     if (geo->bad) {  // grid rejected: the later kernels of this move return at once
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flag, kFlagRejected);
         return;  // (uniform over the block)
     }
-    if (D == 3 && !cs.live) return;  // (2D: every thread reaches the block's barrier)
     const int64_t c = cs.live ? cs.c : 0;
     const int kb = cs.kb, ke = cs.ke;  // slot chunk(s) kb..ke-1
     const int64_t C = g.C;
@@ -1135,7 +1138,7 @@ __global__ void __launch_bounds__(kClassifyMax)
         if (mv8) cw[(k0 >> 3) * C] = word;
         mov |= (uint64_t)mv8 << k0;
         // the chunk's staged movers, packed after the cell's earlier ones
-        const int64_t o0 = chunk_record<D>(c, g.K, k0, stg, staged, s_cnt);
+        const int64_t o0 = chunk_record<true>(c, g.K, k0, stg, staged, s_cnt);
         if (cs.live) {
             if (D == 2 && nargs == 1) {  // 16-byte records (the scatter reads them whole)
 #pragma unroll
@@ -1159,10 +1162,7 @@ __global__ void __launch_bounds__(kClassifyMax)
     }
     if (!cs.live) return;
     store_cell_words(ms, c, kb, ke, g.K, occ, mov);
-    if constexpr (D == 3)
-        ms.stm[c] = staged;
-    else
-        reinterpret_cast<uint8_t*>(ms.stm + c)[kb >> 3] = (uint8_t)(staged >> kb);
+    reinterpret_cast<uint8_t*>(ms.stm + c)[kb >> 3] = (uint8_t)(staged >> kb);
 }
 
 ArgPtrs argptrs(double* const* a, int n) {
@@ -1242,10 +1242,11 @@ int run_move(jp_ctx* ctx, const jp_cellgrid* gc, T* const* coords, uint8_t* inde
     // thread, no chunk's loads wait for an earlier chunk's stores)
     const int span = slot_span(ctx, blocks_for(C), g.K, 6144);
     const dim3 cgrid((unsigned)blocks_for(C), (unsigned)((g.K + span - 1) / span));
-    // classify (classify_slots): 2D a workgroup of one wavefront per 8-slot
-    // chunk for each 64 cells, 3D a thread per cell
-    const unsigned kblocks = (unsigned)(D == 2 ? (C + 63) / 64 : blocks_for(C));
-    const unsigned kthreads = (unsigned)(D == 2 ? 64 * ((g.K + 7) / 8) : kBlock);
+    // classify (classify_slots): a workgroup of one wavefront per 8-slot chunk
+    // for each 64 cells (2D, 3D split), else a thread per cell
+    const bool chunks = D == 2 || !std::is_same<T, double>::value;
+    const unsigned kblocks = (unsigned)(chunks ? (C + 63) / 64 : blocks_for(C));
+    const unsigned kthreads = (unsigned)(chunks ? 64 * ((g.K + 7) / 8) : kBlock);
     if constexpr (std::is_same<T, double>::value) {
         CPtr3 cc{{nullptr, nullptr, nullptr}};
         for (int d = 0; d < D; ++d) cc.p[d] = coords[d];
