@@ -440,12 +440,14 @@ __device__ inline bool interp_tile_f64(const TileLds<D>& L, int c, const double 
 
 // B: slot planes per chunk; FLAGS: particles whose step leaves the windows
 // are flagged (flags[slot], written for every slot of the tile's planes) for
-// k_advect_flagged instead of being advected inline
+// k_advect_flagged instead of being advected inline, and the workgroup
+// records whether it flagged any (any[workgroup])
 template <int D, int M, int B, bool FLAGS>
 __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VPtrs<D> V,
                                                         Ptr3 coords,
                                                         const uint8_t* __restrict__ index,
-                                                        uint8_t* __restrict__ flags, RKParams P,
+                                                        uint8_t* __restrict__ flags,
+                                                        uint8_t* __restrict__ any, RKParams P,
                                                         int span) {
     using TL = F64Tile<D>;
     using WN = F64Win<D>;
@@ -543,6 +545,7 @@ __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VP
     __shared__ uint16_t s_items[kBlock / 64][B * 64];
     uint16_t* items = s_items[tid >> 6];
     const int lane = tid & 63, wbase = tid & ~63;
+    bool missed = false;
     for (int k0 = kb; k0 < ke; k0 += B) {
         if constexpr (FLAGS) {  // every slot of the chunk: no miss yet
 #pragma unroll
@@ -585,6 +588,7 @@ __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VP
             if (!win) {
                 if constexpr (FLAGS) {
                     flags[e] = 1;  // the flagged pass advects it
+                    missed = true;
                     continue;
                 } else {
                     AxisGuess<D> Gs;
@@ -596,27 +600,54 @@ __global__ void __launch_bounds__(kBlock) k_advect_tile(Grid<D> g, Vel<D> vg, VP
             for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
         }
     }
+    if constexpr (FLAGS) {
+        const int m = __syncthreads_or(missed ? 1 : 0);
+        if (tid == 0) any[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = m ? 1 : 0;
+    }
 }
 
-// the flagged particles of the tiled kernel, one thread per slot as in
-// k_advect_slot
+// the flagged particles of the tiled kernel: the same grid of tiles and slot
+// groups, a workgroup returning at once unless its tile flagged some; then
+// one thread per cell of the tile walks its slots, advecting the flagged ones
+// as k_advect_slot does
 template <int D, int M>
 __global__ void __launch_bounds__(kBlock) k_advect_flagged(Grid<D> g, Vel<D> vg, VPtrs<D> V,
                                                            Ptr3 coords,
                                                            const uint8_t* __restrict__ flags,
-                                                           RKParams P) {
-    int64_t c, I[D];
-    if (!row_cell<D>(g, c, I)) return;
-    const int64_t e = c + g.C * (int64_t)blockIdx.y;
-    if (!flags[e]) return;
+                                                           const uint8_t* __restrict__ any,
+                                                           RKParams P, int span) {
+    using TL = F64Tile<D>;
+    if (!any[blockIdx.x + (int64_t)gridDim.x * blockIdx.y]) return;
+    int64_t I[D];
+    {
+        int64_t b = blockIdx.x;
+        int r = threadIdx.x;
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t nt = (g.ni[d] + TL::n[d] - 1) / TL::n[d];
+            I[d] = (b % nt) * TL::n[d] + r % TL::n[d];
+            b /= nt;
+            r /= TL::n[d];
+            in = in && I[d] < g.ni[d];
+        }
+        if (!in) return;
+    }
+    const int kb = blockIdx.y * span;
+    const int ke = kb + span < g.K ? kb + span : g.K;
+    const int64_t c = lin<D>(g.ni, I);
     AxisGuess<D> Gs;
     axis_guess<D>(vg, Gs);
-    double p0[D], out[D];
+    for (int k = kb; k < ke; ++k) {
+        const int64_t e = c + g.C * (int64_t)k;
+        if (!flags[e]) continue;
+        double p0[D], out[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) p0[d] = coords.p[d][e];
-    advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
+        for (int d = 0; d < D; ++d) p0[d] = coords.p[d][e];
+        advect_slot<D, M>(Gs, vg, V, P, p0, I, out);
 #pragma unroll
-    for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
+        for (int d = 0; d < D; ++d) coords.p[d][e] = out[d];
+    }
 }
 
 // semilagrangian_advection! (backtracking.jl:15-101 and the MQS / LinP forms):
@@ -721,17 +752,21 @@ int launch(jp_ctx* ctx, const Grid<D>& G, const Vel<D>& vg, const VPtrs<D>& V, P
             const dim3 grid((unsigned)tiles, (unsigned)((G.K + span - 1) / span));
             if constexpr (D == 2) {
                 k_advect_tile<D, M, 8, false><<<grid, kBlock, 0, ctx->stream>>>(
-                    G, vg, V, cp, index, nullptr, P, span);
+                    G, vg, V, cp, index, nullptr, nullptr, P, span);
                 JP_LAUNCHED(ctx);
             } else {
-                int rc = ensure_scratch(ctx, (size_t)G.C * G.K);
+                // scratch: a flag per slot | a flag per workgroup
+                const size_t nf = ((size_t)G.C * G.K + 255) & ~(size_t)255;
+                const size_t nwg = (size_t)grid.x * grid.y;
+                int rc = ensure_scratch(ctx, nf + nwg);
                 if (rc) return rc;
                 uint8_t* flags = static_cast<uint8_t*>(ctx->scratch);
+                uint8_t* any = flags + nf;
                 k_advect_tile<D, M, 4, true><<<grid, kBlock, 0, ctx->stream>>>(
-                    G, vg, V, cp, index, flags, P, span);
+                    G, vg, V, cp, index, flags, any, P, span);
                 JP_LAUNCHED(ctx);
-                const dim3 sgrid((unsigned)row_blocks<D>(G), (unsigned)G.K);
-                k_advect_flagged<D, M><<<sgrid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, flags, P);
+                k_advect_flagged<D, M><<<grid, kBlock, 0, ctx->stream>>>(G, vg, V, cp, flags,
+                                                                        any, P, span);
                 JP_LAUNCHED(ctx);
             }
             return JP_OK;
