@@ -2,6 +2,7 @@
 #include "jp_ctx.hpp"
 
 #include <atomic>
+#include <mutex>
 
 #include <cstring>
 
@@ -247,6 +248,62 @@ int stats_end(jp_ctx* ctx, jp_stats* host) {
     host->injected += tmp.injected;
     host->orphan += tmp.orphan;
     return JP_OK;
+}
+
+namespace {
+struct OrderedGate {
+    std::mutex m;
+    hipEvent_t ev[64] = {};
+    hipStream_t last[64] = {};
+    bool used[64] = {};
+    bool multi[64] = {};  // a second stream has used ordered launches here
+};
+OrderedGate& ordered_gate() {
+    static OrderedGate g;  // process lifetime (the events are never destroyed)
+    return g;
+}
+}  // namespace
+
+int ordered_gate_enter(jp_ctx* ctx) {
+    if (ctx->capturing) return JP_OK;
+    if (ctx->device < 0 || ctx->device >= 64) return fail(ctx, JP_ERR_STATE, "device index%s");
+    OrderedGate& g = ordered_gate();
+    g.m.lock();
+    const int d = ctx->device;
+    hipError_t e = hipSuccess;
+    if (g.used[d] && g.last[d] != ctx->stream) {
+        if (g.multi[d]) {
+            // wait for the event recorded after the previous ordered launch
+            // (on its stream, at its launch; valid after that stream is gone)
+            e = hipStreamWaitEvent(ctx->stream, g.ev[d], 0);
+        } else {
+            // the first switch of streams on this device: no event was
+            // recorded (a single stream pays nothing), so wait for the device
+            // to finish what it has, then record from now on
+            e = hipDeviceSynchronize();
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&g.ev[d], hipEventDisableTiming);
+            if (e == hipSuccess) g.multi[d] = true;
+        }
+    }
+    if (e != hipSuccess) {
+        g.m.unlock();
+        return fail(ctx, JP_ERR_HIP, "HIP error %s", hipGetErrorString(e));
+    }
+    return JP_OK;  // (the lock is held until ordered_gate_leave)
+}
+
+int ordered_gate_leave(jp_ctx* ctx) {
+    if (ctx->capturing) return JP_OK;
+    OrderedGate& g = ordered_gate();
+    const int d = ctx->device;
+    // recorded on this stream now, while it exists
+    const hipError_t e = g.multi[d] ? hipEventRecord(g.ev[d], ctx->stream) : hipSuccess;
+    if (e == hipSuccess) {
+        g.last[d] = ctx->stream;
+        g.used[d] = true;
+    }
+    g.m.unlock();
+    return e == hipSuccess ? JP_OK : fail(ctx, JP_ERR_HIP, "HIP error %s", hipGetErrorString(e));
 }
 
 int inject_wait_report(jp_ctx* ctx) {

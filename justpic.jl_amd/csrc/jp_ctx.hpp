@@ -114,6 +114,17 @@ inline int* dinject(jp_ctx* ctx) {
 }
 // report (and clear) such a timeout as JP_ERR_STATE (synchronises)
 int inject_wait_report(jp_ctx* ctx);
+// Ordered injections on one device run one at a time: each one's wavefronts
+// wait on each other and are sized to what the device holds at once, so two
+// at once on different streams could each hold slots the other's waits need.
+// gate_enter (before the launch) makes the context stream wait for the
+// previous ordered launch on the device when that was on another stream;
+// gate_leave (after it) records an event on this stream (now, while it
+// exists) once more than one stream has used ordered launches on the device
+// (until then a switch of streams waits for the device instead).  Not
+// applied under graph capture.
+int ordered_gate_enter(jp_ctx* ctx);
+int ordered_gate_leave(jp_ctx* ctx);
 // report one left by asynchronous semilagrangian calls
 int range_sticky(jp_ctx* ctx);
 

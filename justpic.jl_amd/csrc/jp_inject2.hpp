@@ -23,7 +23,8 @@
 // The reference runs the cells in four colour launches (offsets (1,1),
 // (1,2), (2,1), (2,2), injection.jl:28-35) because a cell reads its 3x3
 // neighbourhood and writes itself.  ORDERED runs all four colour lists in
-// one launch: a listed cell first waits until every listed neighbour of an
+// one launch (one at a time per device: ordered_gate_enter/leave): a listed
+// cell first waits until every listed neighbour of an
 // earlier colour has published its result (per-cell state word: 0 not
 // listed, 1 listed, 2 done), which gives every read the value of the colour
 // order.  A later-colour listed neighbour waits for this cell in turn, so it
@@ -432,10 +433,14 @@ int launch_inject2(jp_ctx* ctx, Inject2 a, const InjectLists& L, int64_t cap) {
         a.cap = cap;
         a.state = L.state;
         a.timeout = dinject(ctx);
+        int rc = ordered_gate_enter(ctx);
+        if (rc) return rc;
         k_inject2<SPLIT, true>
             <<<(unsigned)(want < most ? want : most), kBlock, 0, ctx->stream>>>(a);
-        JP_LAUNCHED(ctx);
-        return JP_OK;
+        const hipError_t e = hipGetLastError();
+        rc = ordered_gate_leave(ctx);
+        if (e != hipSuccess) return fail(ctx, JP_ERR_HIP, "HIP error %s", hipGetErrorString(e));
+        return rc;
     }
     for (int ph = 0; ph < 4; ++ph) {
         a.list = L.list + (int64_t)ph * cap;

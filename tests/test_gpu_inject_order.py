@@ -126,3 +126,35 @@ def test_inject_calls_in_a_row(ctx):
         assert st_d == st_o
         assert_state_equal(p, hp, exact=True)
         assert np.array_equal(Td.data.flat(), T, equal_nan=True)
+
+
+def test_inject_two_streams(ctx):
+    """Two contexts (two streams) inject at once with no synchronisation
+    between the calls: the ordered launches on one device run one after the
+    other (the second waits for the first), and each result is the
+    oracle's."""
+    say("two streams")
+    K, minx = 16, 8
+    ctxs = [jp.Context(0, synchronous=False) for _ in range(2)]
+    runs = []
+    for i, cx in enumerate(ctxs):
+        hp = clustered_state(128, 96, K, seed=40 + i, holes=30)
+        rng = np.random.default_rng(i)
+        T = np.where(hp.index == 1, rng.random(hp.S), np.nan)
+        p = to_dev(cx, hp, min_xcell=minx, seed=3 + i)
+        Td = jp.CellArray(cx, hp.ni, hp.K)
+        Td.copy_from(T)
+        cx.sync()
+        runs.append((cx, hp, T, p, Td, 3 + i))
+    for _ in range(2):  # two rounds: each context's second call follows the other's
+        for cx, hp, T, p, Td, seed in runs:
+            jp.inject_particles_(p, (Td,))
+    for cx, *_ in runs:
+        cx.sync()
+    for cx, hp, T, p, Td, seed in runs:
+        st1 = O.inject_particles(hp, [T], minx, seed=seed, counter=0)
+        st2 = O.inject_particles(hp, [T], minx, seed=seed, counter=1)
+        say("oracle", st1, st2)
+        assert st1["injected"] > 500
+        assert_state_equal(p, hp, exact=True)
+        assert np.array_equal(Td.data.flat(), T, equal_nan=True)
