@@ -678,9 +678,11 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
     constexpr int R = RX * RY;
     static_assert(((RX + 2) / 3) * ((RY + 2) / 3) <= BS, "one cell per thread and phase");
     __shared__ uint64_t s_occ[R], s_mov[R], s_arr[R], s_tch[R];
-    // the code bytes of the cell each thread is processing, word j of thread t
-    // at s_code[j][t] (a lane's byte reads hit banks 8 bytes apart)
-    __shared__ uint64_t s_code[8][BS];
+    // the code bytes of the cell each thread is processing: a row of 68
+    // bytes per thread (slot k at byte k; 17 dwords, so the lanes' rows start
+    // on different banks)
+    constexpr int kRow = 68;
+    __shared__ uint32_t s_code[BS * kRow / 4];
     if (*flag) return;  // far jump or rejected grid: nothing moves here
     const int n0 = (int)g.ni[0], n1 = (int)g.ni[1];
     const int ntx = (n0 + TX - 1) / TX;
@@ -709,7 +711,7 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
     const uint32_t C32 = (uint32_t)g.C;
     const uint64_t full = K == 64 ? ~0ull : ((1ull << K) - 1);
     const int nw = (K + 7) >> 3;
-    const uint8_t* __restrict__ sc = reinterpret_cast<const uint8_t*>(&s_code[0][threadIdx.x]);
+    const uint8_t* __restrict__ sc = reinterpret_cast<const uint8_t*>(s_code) + threadIdx.x * kRow;
     int n_moved = 0, n_del = 0, n_drop = 0;
     // the colour-p cells whose processing is exact in the region: x = vx +
     // 3i (i < cx), y = vy + 3j (j < cy)
@@ -723,22 +725,21 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
         cx = vx <= bx ? (bx - vx) / 3 + 1 : 0;
         cy = vy <= by ? (by - vy) / 3 + 1 : 0;
     };
-    auto load_codes = [&](uint32_t S32, uint64_t (&w)[8], uint64_t& st) {
+    auto load_codes = [&](uint32_t S32, uint64_t (&w)[8]) {
         const uint64_t* __restrict__ cp = reinterpret_cast<const uint64_t*>(ms.code) + S32;
 #pragma unroll
         for (int j = 0; j < 8; ++j) w[j] = j < nw ? cp[(int64_t)j * C32] : 0ull;
-        st = ms.stm[S32];
     };
     auto cell_at = [&](int vx, int vy, int cx, int t) {
         return (uint32_t)(vx + 3 * (t % cx)) + (uint32_t)n0 * (uint32_t)(vy + 3 * (t / cx));
     };
     // the code words of each thread's first cell of a phase are loaded one
     // phase ahead, so that they are in flight during the phase before
-    uint64_t nxt[8], nstm = 0;  // (and the cell's staged movers: their record index)
+    uint64_t nxt[8];
     {
         int vx, vy, cx, cy;
         box(0, vx, vy, cx, cy);
-        if ((int)threadIdx.x < cx * cy) load_codes(cell_at(vx, vy, cx, threadIdx.x), nxt, nstm);
+        if ((int)threadIdx.x < cx * cy) load_codes(cell_at(vx, vy, cx, threadIdx.x), nxt);
     }
     for (int p = 0; p < kNB<2>; ++p) {
         const int col0 = p / 3, col1 = p % 3;
@@ -747,14 +748,15 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
         if ((int)threadIdx.x < cx * cy) {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (j < nw) s_code[j][threadIdx.x] = nxt[j];
+                if (j < nw) {
+                    s_code[threadIdx.x * (kRow / 4) + 2 * j] = (uint32_t)nxt[j];
+                    s_code[threadIdx.x * (kRow / 4) + 2 * j + 1] = (uint32_t)(nxt[j] >> 32);
+                }
         }
-        const uint64_t stmv = nstm;
         if (p + 1 < kNB<2>) {
             int wx, wy, dx_, dy_;
             box(p + 1, wx, wy, dx_, dy_);
-            if ((int)threadIdx.x < dx_ * dy_)
-                load_codes(cell_at(wx, wy, dx_, threadIdx.x), nxt, nstm);
+            if ((int)threadIdx.x < dx_ * dy_) load_codes(cell_at(wx, wy, dx_, threadIdx.x), nxt);
         }
         const int t = threadIdx.x;  // at most one cell of the colour per thread
         uint64_t pend = 0;
@@ -777,14 +779,18 @@ __global__ void __launch_bounds__(BS) k_mv_tiles2(Grid<2> g, MoveScratch ms,
             }
             uint64_t occC = s_occ[r], arrS = s_arr[r], tchS = s_tch[r];
             uint64_t free_sp = full;  // slots >= starting_point
+            // a mover of this cell: its packed record, the cell's staged
+            // movers below it (the loop visits the cell's own movers in slot
+            // order, each once; the deleted ones are not staged)
+            uint32_t rec = S32 * (uint32_t)K;
             while (pend) {
                 const int ip = __builtin_ctzll(pend);
                 pend &= pend - 1;
-                uint32_t cd = sc[(ip >> 3) * (BS * 8) + (ip & 7)];
+                uint32_t cd = sc[ip];
                 const uint64_t bit = 1ull << ip;
                 occC &= ~bit;  // empty_particle! on the source slot
-                // a mover of this cell: its packed record
-                uint32_t origin = S32 * (uint32_t)K + (uint32_t)__popcll(stmv & (bit - 1));
+                uint32_t origin = rec;
+                rec += (!(arrS & bit) && cd != kDel) ? 1u : 0u;
                 if (arrS & bit) {  // rare: an arrival of this call is re-slotted
                     origin = ms.org[org_at<2>(S32, (uint32_t)ip, C32, (uint32_t)K)];
                     // waited for here, on the rare path: a wait at the loop
