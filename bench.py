@@ -111,7 +111,8 @@ CALLS = ("advection", "move", "inject", "p2g")
 KERNELS = {"advection": "one advection kernel (3D FP64: plus the flagged slot-path pass)",
            "move": "classify, colour phases (2D: one tiled launch; 3D: 3^D launches), "
                    "far-jump check, scatter",
-           "inject": "scan, list join, 2^D colour wavefront kernels",
+           "inject": "scan, list join, one ordered launch of the four colours (2D; 3D: 2^D "
+                     "colour launches)",
            "p2g": "per-cell partial sums and per-node gather (split) or occupancy words "
                   "and per-node gather (FP64)"}
 
@@ -525,6 +526,203 @@ def run_adjoint(jp, args):
     print(json.dumps(res), flush=True)
 
 
+def time_op(ctx, fn, reps, prep=None):
+    """GPU time of one call in microseconds: after a warm-up call, `reps`
+    calls each bracketed by hipEvents on the context stream (`prep`, untimed,
+    before each: restores a state the call changes)."""
+    if prep:
+        prep()
+    fn()
+    ctx.sync()
+    tot = 0.0
+    for _ in range(reps):
+        if prep:
+            prep()
+        e0, e1 = ctx.event(), ctx.event()
+        ctx.record(e0)
+        fn()
+        ctx.record(e1)
+        ctx.sync()
+        tot += ctx.elapsed_ms(e0, e1)
+    return 1000.0 * tot / reps
+
+
+def run_rows(jp, args):
+    """The rows either side of the hot path (inject_particles_phase!,
+    update_phase_ratios!, grid2particle_flip!, the centroid transfers, the
+    CPU CellArray layout, advection_MQS!/advection_LinP!, the three
+    semilagrangian_advection! forms, cell_index), FP64, at the benchmark's
+    512^2 size: the state of the default workload after three RK2 advection +
+    move steps (holes in the slot planes), built by the oracle and copied to
+    the device.  Each row: GPU time per call (events around the call, state
+    restored between calls where the call changes it), its algorithmic bytes
+    and the fraction of the 8 TB/s HBM peak, and the oracle (C, OpenMP where
+    it threads the loop) timed once on the same state."""
+    from oracle import oracle as O
+    from justpic_hip.configs import gaussian_T
+    from justpic_hip.grids import compute_dx
+    n = args.n or 512
+    reps = max(3, min(args.steps, 20))
+    cfg = stream2d_cfg(n)
+    xv, xi, V, dt = cfg["xv"], cfg["xi_vel"], cfg["V"], cfg["dt"]
+    spec = WORKLOADS[2]
+    O.set_threaded(True)
+    log("rows: building the state,", n, "^2")
+    hp = O.HostParticles(xv, spec["max_xcell"], xi)
+    O.init_particles(hp, spec["nxcell"] // 4, args.seed)
+    for _ in range(3):
+        O.advection(hp, V, O.JP_RK2, 0.5, dt)
+        O.move_particles(hp, [])
+    rng = np.random.default_rng(args.seed)
+    act = hp.index == 1
+    P, S, Cc, K = int(act.sum()), hp.S, hp.C, hp.K
+    Nn = (n + 1) ** 2
+    phases = np.where(act, rng.integers(1, 4, S), np.nan).astype(float)
+    Fp = np.where(act, rng.random(S), np.nan)
+    T = gaussian_T(xv)
+    T0 = np.asfortranarray(T + 0.25 * np.sin(7.0 * T))
+    xc = O.centre_axes(hp)
+    mc = np.meshgrid(*[a.x for a in xc], indexing="ij")
+    Fc_host = np.asfortranarray(np.sin(5 * mc[0]) * np.cos(4 * mc[1]))
+
+    ctx = jp.Context(0, synchronous=False)
+    p = jp.particles_from_host(ctx, xi, hp.coords, hp.index, spec["nxcell"], K,
+                               spec["min_xcell"], seed=args.seed)
+
+    def cells(a):
+        ca = jp.CellArray(ctx, p.ni, K)
+        ca.copy_from(a)
+        return ca
+
+    saved = [jp.CellArray(ctx, p.ni, K) for _ in range(2)] + \
+        [jp.CellArray(ctx, p.ni, K, dtype=np.uint8)]
+    for dst, src in zip(saved, list(p.coords) + [p.index]):
+        dst.data.copy_from_device(src.data)
+    ph_d, Fp_d, Fp2_d = cells(phases), cells(Fp), cells(Fp)
+    ph_s, Fp_s = cells(phases), cells(Fp)
+    Vd = tuple(jp.DeviceArray.from_host(ctx, v) for v in V)
+    Td, T0d = jp.DeviceArray.from_host(ctx, T), jp.DeviceArray.from_host(ctx, T0)
+    Fsl = jp.DeviceArray.from_host(ctx, T)
+    Fcd = jp.DeviceArray.from_host(ctx, Fc_host)
+    Fc_out = jp.centroid_field(p)
+    pr = jp.PhaseRatios(ctx, 3, p.ni)
+    bg = jp.BacktrackGrid(ctx, xi, xv)
+    buf = jp.DeviceArray(ctx, (S,))
+    xs = tuple(jp.DeviceArray.from_host(ctx, np.nan_to_num(c, nan=0.5)) for c in hp.coords)
+    ci_out = tuple(jp.DeviceArray(ctx, (S,), np.int64) for _ in range(2))
+    dax = [jp.particles.DeviceAxis(ctx, a) for a in xv]
+    ci_axes = (jp._abi.jp_axis * 2)(*[a.abi() for a in dax])
+    ctx.sync()
+
+    def restore():
+        for dst, src in zip(list(p.coords) + [p.index], saved):
+            dst.data.copy_from_device(src.data)
+        ph_d.data.copy_from_device(ph_s.data)
+        Fp_d.data.copy_from_device(Fp_s.data)
+
+    def host_copy(h):
+        c = O.HostParticles(h.xv, h.K, h.xi_vel)
+        for d in range(h.D):
+            c.coords[d][:] = h.coords[d]
+        c.index[:] = h.index
+        return c
+
+    def cpu(fn):
+        t0 = time.perf_counter()
+        fn()
+        return 1000.0 * (time.perf_counter() - t0)
+
+    Vb = sum(v.size for v in V) * 8
+    ratio_bytes = sum(f.size for f in pr.fields.values()) * 8
+    rk = jp.RungeKutta2()
+    rows = {}
+
+    def row(name, us, nbytes, cpu_ms, note):
+        rows[name] = {"us": us, "bytes": float(nbytes), "GBs": nbytes / us / 1e3,
+                      "hbm_frac": nbytes / us / 1e3 / 8000.0, "cpu_ms": cpu_ms,
+                      "gpu_over_cpu": cpu_ms * 1e3 / us, "bytes_are": note}
+        log(f"rows: {name:28s} {us:9.1f} us  {nbytes / us / 1e3:7.0f} GB/s  cpu {cpu_ms:9.2f} ms")
+
+    us = time_op(ctx, lambda: jp.update_phase_ratios_(pr, p, ph_d), reps)
+    c_ms = cpu(lambda: O.update_phase_ratios(hp, phases, 3))
+    row("update_phase_ratios", us, S + P * 24 + ratio_bytes, c_ms,
+        "mask + coordinates and phase of each particle + the ratio arrays written")
+
+    us = time_op(ctx, lambda: jp.inject_particles_phase_(p, ph_d, (Fp_d,), (Td,)), reps,
+                 prep=restore)
+    h2, ph2, f2 = host_copy(hp), phases.copy(), Fp.copy()
+    c_ms = cpu(lambda: O.inject_particles_phase(h2, ph2, [f2], [T], spec["min_xcell"],
+                                                args.seed, 0))
+    row("inject_particles_phase", us, S + P * 16, c_ms,
+        "the scan (mask + coordinates of each particle); the listed cells' work is extra")
+
+    us = time_op(ctx, lambda: jp.grid2particle_flip_(Fp2_d, Td, T0d, p, 0.5), reps)
+    f2 = Fp.copy()
+    c_ms = cpu(lambda: O.grid2particle_flip(hp, [f2], [T], [T0], 0.5))
+    row("grid2particle_flip", us, S + P * 32 + 16 * Nn, c_ms,
+        "mask + coordinates, field read and written per particle + F, F0")
+
+    us = time_op(ctx, lambda: jp.particle2centroid_(Fc_out, Fp_s, p), reps)
+    c_ms = cpu(lambda: O.particle2centroid(hp, [Fp]))
+    row("particle2centroid", us, S + P * 24 + 8 * Cc, c_ms,
+        "mask + coordinates and field per particle + the centroid field")
+
+    us = time_op(ctx, lambda: jp.centroid2particle_(Fp2_d, Fcd, p), reps)
+    f2 = Fp.copy()
+    c_ms = cpu(lambda: O.centroid2particle(hp, [f2], [Fc_host]))
+    row("centroid2particle", us, S + P * 24 + 8 * Cc, c_ms,
+        "mask + coordinates per particle, field written + the centroid field")
+
+    src = saved[0]
+    us = time_op(ctx, lambda: ctx.check(ctx.lib.jp_cellarray_to_cpu_layout(
+        ctx.h, src.ptr, buf.ptr, Cc, K, 8), "layout"), reps)
+    a_host = hp.coords[0]
+    c_ms = cpu(lambda: np.ascontiguousarray(a_host.reshape(K, Cc).T))
+    row("cellarray_to_cpu_layout", us, 16 * S, c_ms,
+        "one FP64 CellArray read and written (numpy transpose on one thread for the CPU)")
+
+    for name, fn, sch in (("advection_MQS", jp.advection_MQS_, "MQS"),
+                          ("advection_LinP", jp.advection_LinP_, "LinP")):
+        us = time_op(ctx, lambda: fn(p, rk, Vd, dt), reps, prep=restore)
+        h2 = host_copy(hp)
+        c_ms = cpu(lambda: O.advection_scheme(h2, V, O.JP_RK2, 0.5, dt, sch))
+        row(name + " (RK2)", us, S + 2 * P * 16 + Vb, c_ms,
+            "mask + coordinates read and written + the velocity")
+
+    grid = tuple(compute_dx(a) for a in xv)
+    gvi = tuple(tuple(compute_dx(a) for a in g) for g in xi)
+    for name, fn, sch in (("semilagrangian", jp.semilagrangian_advection_, "linear"),
+                          ("semilagrangian_MQS", jp.semilagrangian_advection_MQS_, "MQS"),
+                          ("semilagrangian_LinP", jp.semilagrangian_advection_LinP_, "LinP")):
+        us = time_op(ctx, lambda: fn(Fsl, Td, rk, Vd, xi, bg, 0.5 * dt), reps)
+        F2 = [T.copy()]
+        c_ms = cpu(lambda: O.semilagrangian(grid, gvi, V, F2, [T], O.JP_RK2, 0.5, 0.5 * dt,
+                                            sch))
+        row(name + " (RK2)", us, 16 * Nn + Vb, c_ms,
+            "one field read (at the node) and written per node + the velocity")
+
+    us = time_op(ctx, lambda: ctx.check(ctx.lib.jp_cell_index(
+        ctx.h, 2, ci_axes, S, jp.api._ptrs(xs), jp.api._ptrs(ci_out)), "cell_index"), reps)
+    pts = [np.nan_to_num(c, nan=0.5) for c in hp.coords]
+    xa = [np.asarray(a.x) for a in xv]
+    # (the oracle's cell_index is one point per call; the CPU figure is its
+    # formula vectorised by numpy, checked against it on a sample first)
+    ref = [O.cell_index(float(v), xa[0]) for v in pts[0][:1000]]
+    vec = lambda x, a: (np.abs(np.trunc((x - a[0]) / abs(a[1] - a[0]))) + 1).astype(np.int64)
+    assert np.array_equal(vec(pts[0][:1000], xa[0]), ref)
+    c_ms = cpu(lambda: [vec(x, a) for x, a in zip(pts, xa)])
+    row("cell_index (2 x S points)", us, 2 * S * 16, c_ms,
+        "two coordinate arrays read, two int64 index arrays written (CPU: the formula "
+        "vectorised by numpy, one thread)")
+    nthr = O.threads()
+    O.set_threaded(False)
+    out = {"case": "rows", "workload": f"2D {n}^2 cells, nxcell 24, max_xcell 48, FP64, the "
+                                      "state after three RK2 advection + move steps",
+           "particles": P, "slots": S, "reps": reps, "cpu_threads": nthr,
+           "cpu": cpu_model(), "rows": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,7 +736,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the other form and the error check")
     ap.add_argument("--check-steps", type=int, default=50)
-    ap.add_argument("--case", choices=("step", "step3d", "ensemble", "adjoint"),
+    ap.add_argument("--case", choices=("step", "step3d", "ensemble", "adjoint", "rows"),
                     default="step",
                     help="step: the 2D 512^2 workload (default); step3d: 3D RK4 128^3; "
                          "ensemble: batched 2D grids; adjoint: gradient of the 256^2 "
@@ -583,6 +781,9 @@ def main():
         return
     if args.case == "adjoint":
         run_adjoint(jp, args)
+        return
+    if args.case == "rows":
+        run_rows(jp, args)
         return
     D = 3 if args.case == "step3d" else 2
     spec = dict(WORKLOADS[D])

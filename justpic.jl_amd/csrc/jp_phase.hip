@@ -64,6 +64,29 @@ __device__ inline bool any_nan(const double (&p)[D]) {
     return r;
 }
 
+// The slots of cell c in order, body(p, phase) for each: the coordinates and
+// phase of 8 slots loaded together before the bodies run (one round trip per
+// 8 slots instead of one per slot; the bodies, and so every sum, keep the
+// slot order).  Empty slots hold NaN coordinates; past K the loads are
+// skipped and the coordinates are NaN.
+template <int D, typename Body>
+__device__ inline void for_slots(const Grid<D>& g, const CPtr3& coords,
+                                 const double* __restrict__ phases, int64_t c, Body&& body) {
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+        double pb[8][D], phb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool in = k0 + j < g.K;
+            const int64_t e = c + g.C * (k0 + j);
+#pragma unroll
+            for (int d = 0; d < D; ++d) pb[j][d] = in ? coords.p[d][e] : __builtin_nan("");
+            phb[j] = in ? phases[e] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) body(pb[j], phb[j]);
+    }
+}
+
 // particles of cell c within half a cell (<=) of m, in slot order; the
 // cell's vertex spacing is returned in di/inv (Julia keeps the loop's last
 // assignment for the faces' boundary pass)
@@ -78,19 +101,14 @@ __device__ inline void gather_half(const Grid<D>& g, const CPtr3& coords,
         di[d] = g.xv[d].dx[J[d]];
         inv[d] = g.xv[d].idx[J[d]];
     }
-    for (int k = 0; k < g.K; ++k) {
-        const int64_t e = c + g.C * k;
-        double p[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-        const double ph = phases[e];
-        if (any_nan<D>(p)) continue;
+    for_slots<D>(g, coords, phases, c, [&](const double (&p)[D], double ph) {
+        if (any_nan<D>(p)) return;
         bool in = true;
 #pragma unroll
         for (int d = 0; d < D; ++d) in = in && (fabs(p[d] - m[d]) <= di[d] / 2.0);
-        if (!in) continue;
+        if (!in) return;
         add_phase<NP>(w, nph, bilinear_weight<D>(m, p, inv), ph);
-    }
+    });
 }
 
 // phase_ratios_center_kernel! (src/PhaseRatios/centers.jl:14-30): the
@@ -111,15 +129,10 @@ __global__ void __launch_bounds__(kBlock) k_ratio_center(Grid<D> g, CAxes<D> xc,
     }
 #pragma unroll
     for (int j = 0; j < NP; ++j) w[j] = 0.0;
-    for (int k = 0; k < g.K; ++k) {
-        const int64_t e = c + g.C * k;
-        double p[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-        const double ph = phases[e];
-        if (p[0] != p[0]) continue;
+    for_slots<D>(g, coords, phases, c, [&](const double (&p)[D], double ph) {
+        if (p[0] != p[0]) return;
         add_phase<NP>(w, nph, bilinear_weight<D>(m, p, inv), ph);
-    }
+    });
     normalise_write<NP>(w, nph, out, g.C, c, false);
 }
 
@@ -169,19 +182,14 @@ __global__ void __launch_bounds__(kBlock) k_ratio_vertex(Grid<D> g, CPtr3 coords
             di[d] = g.xv[d].dx[J[d]];
             inv[d] = g.xv[d].idx[J[d]];
         }
-        for (int k = 0; k < g.K; ++k) {
-            const int64_t e = c + g.C * k;
-            double p[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-            const double ph = phases[e];
-            if (any_nan<D>(p)) continue;
+        for_slots<D>(g, coords, phases, c, [&](const double (&p)[D], double ph) {
+            if (any_nan<D>(p)) return;
             bool far = false;
 #pragma unroll
             for (int d = 0; d < D; ++d) far = far || (fabs(p[d] - v[d]) >= di[d] / 2.0);
-            if (far) continue;
+            if (far) return;
             add_phase<NP>(w, nph, bilinear_weight<D>(v, p, inv), ph);
-        }
+        });
     }
     normalise_write<NP>(w, nph, out, NN, node, false);
 }
@@ -244,19 +252,14 @@ __global__ void __launch_bounds__(kBlock) k_ratio_face(Grid<D> g, CAxes<D> xc, C
     for (int j = 0; j < NP; ++j) w2[j] = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) f[d] = xcen[d] - (di[d] * (d == dim ? 1.0 : 0.0)) / 2.0;
-    for (int k = 0; k < g.K; ++k) {
-        const int64_t e = c + g.C * k;
-        double p[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = coords.p[d][e];
-        const double ph = phases[e];
-        if (any_nan<D>(p)) continue;
+    for_slots<D>(g, coords, phases, c, [&](const double (&p)[D], double ph) {
+        if (any_nan<D>(p)) return;
         bool in = true;
 #pragma unroll
         for (int d = 0; d < D; ++d) in = in && (fabs(p[d] - f[d]) <= di[d] / 2.0);
-        if (!in) continue;
+        if (!in) return;
         add_phase<NP>(w2, nph, bilinear_weight<D>(f, p, inv), ph);
-    }
+    });
     normalise_write<NP>(w2, nph, out, L, lin_dims<D>(I, dims), true);
 }
 
