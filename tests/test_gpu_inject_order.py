@@ -10,7 +10,10 @@ and split: masks, positions and the copied field), with the default number
 of persistent wavefronts, with 64 of them (each takes many cells, so a
 wavefront waits on cells that other wavefronts take later in their own
 order) and with one workgroup of four.  Two calls in a row on one context
-check that the per-cell state of one call does not leak into the next.
+check that the per-cell state of one call does not leak into the next; two
+contexts inject at once; edge cases: 2 x 2 cells, two rows or columns, every
+cell listed (all empty, with orphans), K = 64, and a wavefront count far
+above what the device holds at once.
 """
 import contextlib
 
@@ -158,3 +161,66 @@ def test_inject_two_streams(ctx):
         assert st1["injected"] > 500
         assert_state_equal(p, hp, exact=True)
         assert np.array_equal(Td.data.flat(), T, equal_nan=True)
+
+
+def empty_state(n0, n1, K, seed, fill=0.0):
+    """n0 x n1 cells holding a fraction `fill` of particles (0: all empty)."""
+    xv = (Axis(np.arange(n0 + 1) / n0, "range"), Axis(np.arange(n1 + 1) / n1, "range"))
+    hp = O.HostParticles(xv, K, staggered_grid(xv))
+    clear(hp)
+    rng = np.random.default_rng(seed)
+    for i in range(n0):
+        for j in range(n1):
+            for s in range(K):
+                if rng.random() < fill:
+                    place(hp, (i, j), s, ((i + rng.uniform(0.02, 0.98)) / n0,
+                                          (j + rng.uniform(0.02, 0.98)) / n1))
+    return hp
+
+
+EDGE = [
+    # (n0, n1, K, min_xcell, fill)
+    (2, 2, 16, 8, 0.1),     # four cells, every one a neighbour of the others
+    (33, 2, 12, 8, 0.2),    # two rows
+    (2, 29, 12, 8, 0.2),    # two columns
+    (40, 24, 16, 12, 0.0),  # every cell listed (all empty): the longest chains
+    (24, 20, 64, 32, 0.3),  # K = 64
+]
+
+
+@pytest.mark.parametrize("case", EDGE)
+def test_inject_one_launch_edges(ctx, case):
+    n0, n1, K, minx, fill = case
+    say("edges", case)
+    hp = empty_state(n0, n1, K, seed=n0 * 7 + n1, fill=fill)
+    rng = np.random.default_rng(2)
+    T = np.where(hp.index == 1, rng.random(hp.S), -1.0 - rng.random(hp.S))
+    p, s, Td, Ts, st_d, st_s = run_pair(ctx, hp, T, minx, 5)
+    q, r, Qd, Qs, sq_d, sq_s = run_pair(ctx, hp, T, minx, 5, phases=1)
+    st_o = O.inject_particles(hp, [T], minx, seed=5, counter=0)
+    say("stats", st_d, st_o)
+    assert st_o["injected"] > 0
+    assert st_d == st_o and st_s == st_o and sq_d == st_o and sq_s == st_o
+    assert_state_equal(p, hp, exact=True)
+    assert np.array_equal(Td.data.flat(), T, equal_nan=True)
+    c1, i1 = split_state(s)
+    c2, i2 = split_state(r)
+    assert np.array_equal(i1, hp.index) and np.array_equal(i2, hp.index)
+    for d in range(2):
+        assert np.array_equal(c1[d], c2[d], equal_nan=True)
+    assert np.array_equal(Ts.data.flat(), Qs.data.flat(), equal_nan=True)
+
+
+def test_inject_one_launch_more_waves_than_resident(ctx):
+    """A wavefront count far above what the device holds at once: the ordered
+    launch is capped at the resident count (every wait ends) and the result
+    is the oracle's."""
+    say("waves above residency")
+    hp = clustered_state(96, 80, 16, seed=5)
+    rng = np.random.default_rng(4)
+    T = np.where(hp.index == 1, rng.random(hp.S), np.nan)
+    p, s, Td, Ts, st_d, st_s = run_pair(ctx, hp, T, 8, 9, waves=1 << 20)
+    st_o = O.inject_particles(hp, [T], 8, seed=9, counter=0)
+    assert st_d == st_o and st_s == st_o
+    assert_state_equal(p, hp, exact=True)
+    assert np.array_equal(Td.data.flat(), T, equal_nan=True)
