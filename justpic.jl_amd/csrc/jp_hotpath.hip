@@ -402,6 +402,29 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
     double p[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) p[d] = act ? coords.p[d][e] : 0.0;
+    // 2D: the neighbour cells' masks and coordinates, loaded once for all of
+    // this cell's searches (they do not change while it injects: the cells
+    // of one colour are two apart); 3D loads them per search, in batches
+    uint8_t m9[D == 2 ? NB : 1];
+    double q9[D == 2 ? NB : 1][D];
+    if constexpr (D == 2) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            int64_t J[D];
+            bool ok = own && u != CENTER;
+            int r = u;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                J[d] = I[d] + (r % 3) - 1;
+                r /= 3;
+                ok = ok && J[d] >= 0 && J[d] < g.ni[d];
+            }
+            const int64_t e2 = ok ? lin<D>(g.ni, J) + C * lane : 0;
+            m9[u] = ok ? index[e2] : (uint8_t)0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) q9[u][d] = ok ? coords.p[d][e2] : 0.0;
+        }
+    }
     for (int q = 0; q < NQ; ++q) {
         double vert[D];
 #pragma unroll
@@ -466,9 +489,16 @@ __global__ void __launch_bounds__(kBlock) k_inject_wave(Grid<D> g, Ptr3 coords, 
 #pragma unroll
                 for (int u = 0; u < NBB; ++u) {
                     const bool other = a[u] && nb0 + u != CENTER;
-                    m[u] = other ? index[e2[u]] : (uint8_t)0;
+                    if constexpr (D == 2) {
+                        m[u] = other ? m9[nb0 + u] : (uint8_t)0;
 #pragma unroll
-                    for (int d = 0; d < D; ++d) pq[u][d] = other ? coords.p[d][e2[u]] : p[d];
+                        for (int d = 0; d < D; ++d) pq[u][d] = other ? q9[nb0 + u][d] : p[d];
+                    } else {
+                        m[u] = other ? index[e2[u]] : (uint8_t)0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d)
+                            pq[u][d] = other ? coords.p[d][e2[u]] : p[d];
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < NBB; ++u)
