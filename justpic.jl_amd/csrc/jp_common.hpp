@@ -240,10 +240,6 @@ __device__ inline uint64_t active_slots_in(const uint8_t* __restrict__ index, in
     return bits;
 }
 
-// Appends c to the list segment of its wavefront and colour without atomics:
-// segment (q, wave) holds up to 64 cells at seg[(q*nwave + wave)*64 + i] and
-// its length at cnt[q*nwave + wave]; a compaction kernel joins the segments.
-// Every lane of the wavefront must call it (want = false: nothing to add).
 // Nearest-neighbour search of the injection on squared distances.
 // wave_argmin: lexicographic minimum of (s, order) over the wavefront.
 __device__ inline void wave_argmin(double& best, int& border) {
@@ -287,6 +283,13 @@ __device__ inline void resolve_ties(const double (&s9)[NB], double best, int& bo
 // the bound)
 constexpr double kPrune = 1.0 + 0x1p-40;
 
+// Appends c to the list segment of its wavefront and colour without atomics:
+// segment (q, wave) holds up to 64 cells at seg[(q*nwave + wave)*64 + i] and
+// its length at cnt[q*nwave + wave]; a compaction kernel joins the segments
+// in cell order, which the ordered injection relies on for speed (a cell's
+// neighbours of earlier colours are dealt early; appending with atomics, in
+// whatever order wavefronts arrive, measured 85 -> 125 us at 512^2).
+// Every lane of the wavefront must call it (want = false: nothing to add).
 template <int NQ>
 __device__ inline void wave_segment(bool want, int col, int32_t c, int64_t wave, int64_t nwave,
                                     int32_t* __restrict__ seg, int32_t* __restrict__ cnt) {
